@@ -1,0 +1,145 @@
+"""Throughput of the FP64 residual + RK3 path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--flux 1|2|3]
+
+A step is one pseudo-time iteration (save_old, time_step, 3 x [residual,
+RK stage], CL/CD, residual norms) of the transonic Euler workload on the
+seeded 513 x 129 NACA0012 O-grid (66 048 vertices, 131 072 triangles,
+197 120 edges; M = 0.8, alpha = 1.25 deg, Roe flux, limited MUSCL, CFL 1).
+The grid and state are resident in HBM before timing starts.  Prints one
+JSON line with the throughput in vertex-iterations/s, the roofline of the
+dominant kernel (algorithmic bytes per launch / its HIP-event duration on
+the solver's stream) and the CPU baseline (the oracle, single core, on a
+bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, vendor figure
+
+
+def algorithmic_bytes(kernel: str, npt: int, nt: int, ne: int) -> float | None:
+    """Compulsory traffic of one launch (each array touched once, gathers
+    counted once per vertex; FP64 8 B, int32 4 B):
+      gradient pass  G = 12 nt + (16 + 32 + 8 + 64) np
+      edge-flux pass F = 24 ne + (16 + 32 + 64 + 32) np
+      update pass    U = (32 + 32 + 32 + 8 + 8 + 32) np
+      save_old + time_step = (64 + 48) np"""
+    return {"k_grad": 12 * nt + 120 * npt, "k_flux": 24 * ne + 144 * npt,
+            "k_update": 144 * npt, "k_tstep": 112 * npt}.get(kernel)
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), if a summary for
+    this workload is committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", "traffic_513x129_euler_roe.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(grid, kw, target_s: float = 12.0) -> dict:
+    """The oracle (a serial C restatement of the CPU solver, gcc -O2) on the
+    host: iterations of the same workload from the free stream."""
+    from oracle.oracle import Oracle
+    o = Oracle(grid.coord, grid.elem, grid.ptype, **kw)
+    t0 = time.perf_counter()
+    o.iterate(2)
+    t_it = (time.perf_counter() - t0) / 2
+    n = max(3, min(2000, int(target_s / max(t_it, 1e-6))))
+    o2 = Oracle(grid.coord, grid.elem, grid.ptype, **kw)
+    t0 = time.perf_counter()
+    h, _ = o2.iterate(n)
+    dt = time.perf_counter() - t0
+    return {"value": grid.np * h.shape[0] / dt, "unit": "vertex-iterations/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{h.shape[0]} iterations of the same 513x129 Euler/Roe workload from the "
+                      f"free stream, serial C oracle (gcc -O2 -ffp-contract=off) on "
+                      f"{platform.processor() or platform.machine()}, {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--flux", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the workload is a single-GPU solve and this build "
+              "has no multi-GPU path (independent solves shard one per GPU).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    from fvm2d_amd import mesh
+    from fvm2d_amd.solver import Inputs, Solver
+
+    g = mesh.naca0012_513x129()
+    kw = dict(iflow=1, iflux=a.flux, ilimit=1, mach_inf=0.8, aoa_deg=1.25, cfl=1.0,
+              maxiter=10**9, minres=0.0)
+    s = Solver(g, Inputs.from_kwargs(**kw))
+    dims = s.geo.dims
+    npt, nt, ne = dims["np"], dims["nt"], dims["ne"]
+
+    s.iterate(a.warmup)
+    t0 = time.perf_counter()
+    hist, status = s.iterate(a.steps)          # returns after a device synchronize
+    elapsed = time.perf_counter() - t0
+    if hist.shape[0] != a.steps:
+        print(f"solve stopped early: status {status}", file=sys.stderr)
+        return 1
+    value = npt * a.steps / elapsed
+
+    roofline = None
+    if not a.no_kernel_timing:
+        ms = s.time_kernels(20)
+        per_iter = {k: v * (3 if k in ("k_grad", "k_flux", "k_update") else 1) for k, v in ms.items()}
+        dom = max(per_iter, key=per_iter.get)
+        B = algorithmic_bytes(dom, npt, nt, ne)
+        if B is not None:
+            achieved = B / (ms[dom] * 1e-3) / 1e9
+            roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": traffic_from_profiles(dom),
+                        "kernel_ms": {k: round(v, 5) for k, v in ms.items()},
+                        "algorithmic_bytes_per_launch": B}
+
+    cpu = None if a.no_cpu else cpu_baseline(g, kw)
+    line = {
+        "metric": "vertex-iters/s", "value": round(value, 1), "unit": "vertex-iterations/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(1e3 * elapsed / a.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "NACA0012 O-grid 513x129 (66048 vertices, 131072 triangles), "
+                               "Euler, %s flux, limited MUSCL, RK3, CFL 1, M=0.8, alpha=1.25, FP64"
+                               % {1: "Roe", 2: "KFVS", 3: "HLLC"}[a.flux],
+                   "np": npt, "nt": nt, "ne": ne, "precision": "fp64"},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "cl": float(hist[-1, 4]), "cd": float(hist[-1, 5]),
+    }
+    if cpu:
+        line["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    s.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,58 @@
+"""Builds libfvm2d_gpu.so in-tree: HIP kernels for gfx950 + host code.
+
+    python -m fvm2d_amd.build
+
+Flags that matter for results:
+  -ffp-contract=off      no fused multiply-add, so the FP64 path evaluates
+                         every expression exactly as written;
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                         no packed FP32 instructions.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUTDIR = os.path.join(HERE, "_lib")
+LIB = os.path.join(OUTDIR, "libfvm2d_gpu.so")
+ARCH = os.environ.get("FVM2D_ARCH", "gfx950")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+CXX = shutil.which("g++") or "g++"
+
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-pthread"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {os.path.basename(cmd[0])}")
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OUTDIR, exist_ok=True)
+    obj_geom = os.path.join(OUTDIR, "geometry.o")
+    obj_hip = os.path.join(OUTDIR, "fvm2d_gpu.o")
+    _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "geometry.cpp"), "-o", obj_geom])
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "fvm2d_gpu.hip"), "-o", obj_hip])
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, obj_hip, obj_geom,
+          "-pthread"])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

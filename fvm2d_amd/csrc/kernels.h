@@ -1,0 +1,336 @@
+// kernels.h -- point functions of the hot path, shared by every kernel.
+//
+// Each one evaluates its formula in the reference's order of operations, so
+// that, compiled with -ffp-contract=off, the FP64 path reproduces the CPU
+// solver bit for bit wherever no libm transcendental is involved.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fvm {
+
+constexpr double kEps = 1.0e-16;             // EPSILON, src/param.h:105
+
+// COMMON-block values the kernels read (src/param.h:56-142, src/common.h)
+struct Consts {
+    double GAMMA, GAMMA1, M_PI_, prandtl, prandtl_turb;
+    double ALBADA11, ALBADA12, ALBADA21, ALBADA22;
+    double Cb1, Cw1, Cw2, Cv2, Cv11, Cw31, Cw32, kolm2, Cb2Sig1, Cb2Sig2;
+    double q_inf, r_inf, p_inf, T_inf, aoa, u_inf, v_inf, a_inf, SCONST, nut_inf;
+    double rey, cfl, mach_inf, xref, yref, minres;
+    double airk[3], birk[3];
+    int iflow, iflux, ilimit, vortex, sa_compat, hllc_rotate;
+    int iterlast, maxiter, saveinterval, np;
+};
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+// integer powers as binary exponentiation: x**3 = x*(x*x), x**6 = x^2*x^4
+__device__ __forceinline__ double pow3(double x) { double x2 = x * x; return x * x2; }
+__device__ __forceinline__ double pow6(double x) { double x2 = x * x; double x4 = x2 * x2; return x2 * x4; }
+
+struct V4 { double v[4]; };
+__device__ __forceinline__ V4 ld4(const double4 *p, int i)
+{
+    double4 a = p[i];
+    return V4{{a.x, a.y, a.z, a.w}};
+}
+__device__ __forceinline__ void st4(double4 *p, int i, const V4 &a)
+{
+    p[i] = make_double4(a.v[0], a.v[1], a.v[2], a.v[3]);
+}
+
+// van Albada limiter, src/recon.f:55-71
+__device__ __forceinline__ double limit_albada(double ul, double ur)
+{
+    if (ul * ur <= 0.0) return 0.0;
+    double top = (sq(ul) + kEps) * ur + (sq(ur) + kEps) * ul;
+    double bot = sq(ul) + sq(ur) + 2.0 * kEps;
+    return top / bot;
+}
+
+// MUSCL states at the edge midpoint, src/recon.f:15-44
+template <int LIM>
+__device__ __forceinline__ void recon(const Consts &c, double dx, double dy, const V4 &p1,
+                                      const V4 &p2, const V4 &qx1, const V4 &qy1,
+                                      const V4 &qx2, const V4 &qy2, V4 &pl, V4 &pr)
+{
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double dqp = p2.v[i] - p1.v[i];
+        if (LIM) {
+            double dql = c.ALBADA21 * (dx * qx1.v[i] + dy * qy1.v[i]) + c.ALBADA22 * dqp;
+            pl.v[i] = p1.v[i] + 0.5 * limit_albada(dql, dqp);
+            double dqr = c.ALBADA21 * (dx * qx2.v[i] + dy * qy2.v[i]) + c.ALBADA22 * dqp;
+            pr.v[i] = p2.v[i] - 0.5 * limit_albada(dqr, dqp);
+        } else {
+            double dql = dx * qx1.v[i] + dy * qy1.v[i];
+            pl.v[i] = p1.v[i] + 0.5 * (c.ALBADA11 * dql + c.ALBADA12 * dqp);
+            double dqr = dx * qx2.v[i] + dy * qy2.v[i];
+            pr.v[i] = p2.v[i] - 0.5 * (c.ALBADA11 * dqr + c.ALBADA12 * dqp);
+        }
+    }
+}
+
+// Roe flux without entropy fix, src/roe_flux.f:18-112.  false: a^2 <= 0.
+__device__ __forceinline__ bool roe_flux(const Consts &c, double dsx, double dsy, const V4 &L,
+                                         const V4 &R, V4 &f)
+{
+    const double G = c.GAMMA, G1 = c.GAMMA1;
+    double length = sqrt(sq(dsx) + sq(dsy));
+    double ct = dsx / length, st = dsy / length;
+    double rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
+    double al2 = G * pl / rl, ql2 = sq(ul) + sq(vl), hl = al2 / G1 + 0.5 * ql2;
+    double rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
+    double ar2 = G * pr / rr, qr2 = sq(ur) + sq(vr), hr = ar2 / G1 + 0.5 * qr2;
+    double unl = ul * ct + vl * st;
+    double rl12 = sqrt(rl), rr12 = sqrt(rr), rd = 1.0 / (rl12 + rr12);
+    double ua = (ul * rl12 + ur * rr12) * rd;
+    double va = (vl * rl12 + vr * rr12) * rd;
+    double ha = (hl * rl12 + hr * rr12) * rd;
+    double qa2 = sq(ua) + sq(va);
+    double aa2 = G1 * (ha - 0.5 * qa2);
+    if (!(aa2 > 0.0)) return false;
+    double aa = sqrt(aa2);
+    double una = ua * ct + va * st;
+    double vna = -ua * st + va * ct;
+    double l1 = fmin(una - aa, 0.0), l2 = fmin(una, 0.0), l4 = fmin(una + aa, 0.0);
+    double du1 = rr - rl, du2 = rr * ur - rl * ul, du3 = rr * vr - rl * vl;
+    double du4 = (pr - pl) / G1 + 0.5 * (rr * qr2 - rl * ql2);
+    double m1 = (ct * du2 + st * du3 - una * du1) / aa;
+    double m2 = G1 * (du4 - ua * du2 - va * du3 + qa2 * du1) / sq(aa);
+    double a4 = 0.5 * (m1 + m2), a1 = 0.5 * (m2 - m1), a3 = du1 - a1 - a4;
+    double a2 = (st * du2 - ct * du3 + vna * du1) / aa;
+    double a1l1 = a1 * l1, a2l2 = a2 * l2, a3l3 = a3 * l2, a4l4 = a4 * l4;
+    double aact = aa * ct, aast = aa * st;
+    double Fc0 = rl * unl, Fc1 = pl * ct + rl * ul * unl, Fc2 = pl * st + rl * vl * unl,
+           Fc3 = rl * hl * unl;
+    double Fd0 = a1l1 + a3l3 + a4l4;
+    double Fd1 = a1l1 * (ua - aact) + a2l2 * aa * st + a3l3 * ua + a4l4 * (ua + aact);
+    double Fd2 = a1l1 * (va - aast) - a2l2 * aa * ct + a3l3 * va + a4l4 * (va + aast);
+    double Fd3 = a1l1 * (ha - una * aa) + a2l2 * aa * vna + a3l3 * 0.5 * qa2 + a4l4 * (ha + una * aa);
+    f.v[0] = (Fc0 + Fd0) * length;
+    f.v[1] = (Fc1 + Fd1) * length;
+    f.v[2] = (Fc2 + Fd2) * length;
+    f.v[3] = (Fc3 + Fd3) * length;
+    return true;
+}
+
+// kinetic flux-vector splitting, src/kfvs_flux.f:16-78
+__device__ __forceinline__ void kfvs_flux(const Consts &c, double dsx, double dsy, const V4 &L,
+                                          const V4 &R, V4 &f)
+{
+    const double G1 = c.GAMMA1;
+    double length = sqrt(sq(dsx) + sq(dsy));
+    double ct = dsx / length, st = dsy / length;
+    double rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
+    double el = pl / G1 + 0.5 * rl * (sq(ul) + sq(vl));
+    double rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
+    double er = pr / G1 + 0.5 * rr * (sq(ur) + sq(vr));
+    double unl = ul * ct + vl * st, unr = ur * ct + vr * st;
+    double vnl = -ul * st + vl * ct, vnr = -ur * st + vr * ct;
+    double betal = 0.5 * rl / pl, sl = unl * sqrt(betal);
+    double Al = 0.5 * (1.0 + erf(sl));
+    double Bl = 0.5 * exp(-sq(sl)) / sqrt(c.M_PI_ * betal);
+    double betar = 0.5 * rr / pr, sr = unr * sqrt(betar);
+    double Ar = 0.5 * (1.0 - erf(sr));
+    double Br = 0.5 * exp(-sq(sr)) / sqrt(c.M_PI_ * betar);
+    double F0 = rl * (unl * Al + Bl) + rr * (unr * Ar - Br);
+    double F1 = ((pl + rl * sq(unl)) * Al + rl * unl * Bl) + ((pr + rr * sq(unr)) * Ar - rr * unr * Br);
+    double F2 = rl * (unl * Al + Bl) * vnl + rr * (unr * Ar - Br) * vnr;
+    double F3 = ((el + pl) * unl * Al + (el + 0.5 * pl) * Bl) + ((er + pr) * unr * Ar - (er + 0.5 * pr) * Br);
+    f.v[0] = F0 * length;
+    f.v[1] = (ct * F1 - st * F2) * length;
+    f.v[2] = (st * F1 + ct * F2) * length;
+    f.v[3] = F3 * length;
+}
+
+// HLLC (Toro ch. 10), src/hllc_flux.f:18-154.  The reference adds the
+// edge-frame flux to the x-y momentum residual; ROT=1 rotates it back.
+template <int ROT>
+__device__ __forceinline__ void hllc_flux(const Consts &c, double dsx, double dsy, const V4 &L,
+                                          const V4 &R, V4 &f)
+{
+    const double gamma = c.GAMMA, G1 = c.GAMMA1;
+    double length = sqrt(sq(dsx) + sq(dsy));
+    double ct = dsx / length, st = dsy / length;
+    double rl = L.v[0], ul = L.v[1] * ct + L.v[2] * st, vl = -L.v[1] * st + L.v[2] * ct, pl = L.v[3];
+    double al = sqrt(gamma * pl / rl), El = pl / G1 + 0.5 * rl * (sq(ul) + sq(vl));
+    double rr = R.v[0], ur = R.v[1] * ct + R.v[2] * st, vr = -R.v[1] * st + R.v[2] * ct, pr = R.v[3];
+    double ar = sqrt(gamma * pr / rr), Er = pr / G1 + 0.5 * rr * (sq(ur) + sq(vr));
+    double rbar = 0.5 * (rl + rr), abar = 0.5 * (al + ar);
+    double Ppv = 0.5 * (pl + pr - (ur - ul) * rbar * abar);
+    double Upv = 0.5 * (ul + ur - (pr - pl) / (rbar * abar));
+    double prmax = (pl > pr) ? pl : pr, prmin = (pl > pr) ? pr : pl;
+    double Q = prmax / prmin, pstar, ustar;
+    if (Q < 2.0 && prmin < Ppv && Ppv < prmax) {
+        pstar = Ppv; ustar = Upv;
+    } else if (Ppv < prmin) {
+        double z = 0.5 * (gamma - 1.0) / gamma;
+        double a1 = al + ar - 0.5 * (gamma - 1.0) * (ur - ul);
+        double a2 = al / pow(pl, z) + ar / pow(pr, z);
+        pstar = pow(a1 / a2, 1.0 / z);
+        double Plr = pow(pl / pr, z);
+        double b1 = Plr * ul / al + ur / ar + 2.0 * (Plr - 1.0) / (gamma - 1.0);
+        double b2 = Plr / al + 1.0 / ar;
+        ustar = b1 / b2;
+    } else {
+        double po = fmax(0.0, Ppv);
+        double Ml = 2.0 / (gamma + 1.0) / rl, Nl = (gamma - 1.0) * pl / (gamma + 1.0);
+        double gl = sqrt(Ml / (po + Nl));
+        double Mr = 2.0 / (gamma + 1.0) / rr, Nr = (gamma - 1.0) * pr / (gamma + 1.0);
+        double gr = sqrt(Mr / (po + Nr));
+        pstar = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
+        ustar = 0.5 * (ul + ur + (pstar - pr) * gr - (pstar - pl) * gl);
+    }
+    double ql = (pstar <= pl) ? 1.0 : sqrt(1.0 + 0.5 * (gamma + 1.0) * (pstar / pl - 1.0) / gamma);
+    double qr = (pstar <= pr) ? 1.0 : sqrt(1.0 + 0.5 * (gamma + 1.0) * (pstar / pr - 1.0) / gamma);
+    double Sl = ul - al * ql, Ss = ustar, Sr = ur + ar * qr;
+    double F[4];
+    if (0.0 <= Sl) {
+        F[0] = rl * ul; F[1] = pl + rl * sq(ul); F[2] = rl * ul * vl; F[3] = (El + pl) * ul;
+    } else if (Sl <= 0.0 && 0.0 <= Ss) {
+        double fact = rl * (Sl - ul) / (Sl - Ss);
+        double t1 = El / rl + (Ss - ul) * (Ss + pl / rl / (Sl - ul));
+        F[0] = rl * ul + Sl * (fact - rl);
+        F[1] = (pl + rl * sq(ul)) + Sl * (fact * Ss - rl * ul);
+        F[2] = rl * ul * vl + Sl * (fact * vl - rl * vl);
+        F[3] = (El + pl) * ul + Sl * (fact * t1 - El);
+    } else if (Ss <= 0.0 && 0.0 <= Sr) {
+        double fact = rr * (Sr - ur) / (Sr - Ss);
+        double t1 = Er / rr + (Ss - ur) * (Ss + pr / rr / (Sr - ur));
+        F[0] = rr * ur + Sr * (fact - rr);
+        F[1] = (pr + rr * sq(ur)) + Sr * (fact * Ss - rr * ur);
+        F[2] = rr * ur * vr + Sr * (fact * vr - rr * vr);
+        F[3] = (Er + pr) * ur + Sr * (fact * t1 - Er);
+    } else {
+        F[0] = rr * ur; F[1] = pr + rr * sq(ur); F[2] = rr * ur * vr; F[3] = (Er + pr) * ur;
+    }
+    if (ROT) {
+        double fx = ct * F[1] - st * F[2], fy = st * F[1] + ct * F[2];
+        F[1] = fx; F[2] = fy;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f.v[i] = F[i] * length;
+}
+
+// slip wall: pressure force with 3/4-1/4 weights, src/solid_flux.f:10-17
+__device__ __forceinline__ void solid_flux(double2 x1, double2 x2, double p1_, double p2_, V4 &r1,
+                                           V4 &r2)
+{
+    double sx = 0.5 * (x2.y - x1.y);
+    double sy = -0.5 * (x2.x - x1.x);
+    double p1 = 0.75 * p1_ + 0.25 * p2_;
+    double p2 = 0.25 * p1_ + 0.75 * p2_;
+    r1.v[0] = 0.0; r1.v[1] = p1 * sx; r1.v[2] = p1 * sy; r1.v[3] = 0.0;
+    r2.v[0] = 0.0; r2.v[1] = p2 * sx; r2.v[2] = p2 * sy; r2.v[3] = 0.0;
+}
+
+__device__ __forceinline__ void prim2con(const Consts &c, const V4 &p, V4 &u)
+{
+    u.v[0] = p.v[0];
+    u.v[1] = p.v[0] * p.v[1];
+    u.v[2] = p.v[0] * p.v[2];
+    u.v[3] = p.v[3] / c.GAMMA1 + 0.5 * p.v[0] * (sq(p.v[1]) + sq(p.v[2]));
+}
+
+// Steger-Warming farfield flux, src/farfield_flux.f:28-200.  r1/r2 are the
+// contributions to the two end points.
+__device__ inline void farfield_flux(const Consts &c, double cl, double2 x1, double2 x2,
+                                     const V4 &P1, const V4 &P2, V4 &r1, V4 &r2)
+{
+    const double G = c.GAMMA, G1 = c.GAMMA1;
+    double uinf, vinf, q2inf, pinf, rinf;
+    if (c.mach_inf < 1.0 && c.vortex == 1) {
+        double dx = 0.5 * (x1.x + x2.x) - c.xref;
+        double dy = 0.5 * (x1.y + x2.y) - c.yref;
+        double dref = sqrt(sq(dx) + sq(dy));
+        double theta = atan2(dy, dx);
+        double circ = 0.5 * c.q_inf * cl;
+        double fact1 = circ * sqrt(1.0 - sq(c.mach_inf));
+        double fact2 = 2.0 * c.M_PI_ * dref;
+        double fact3 = 1.0 - sq(c.mach_inf * sin(theta - c.aoa));
+        double fact = fact1 / (fact2 * fact3);
+        uinf = c.u_inf + fact * sin(theta);
+        vinf = c.v_inf - fact * cos(theta);
+        q2inf = sq(uinf) + sq(vinf);
+        double fact4 = 1.0 + 0.5 * G1 * (sq(c.q_inf) - q2inf) / sq(c.a_inf);
+        rinf = c.r_inf * pow(fact4, 1.0 / G1);
+        pinf = c.p_inf * pow(rinf / c.r_inf, G);
+    } else {
+        uinf = c.u_inf; vinf = c.v_inf; q2inf = sq(uinf) + sq(vinf);
+        pinf = c.p_inf; rinf = c.r_inf;
+    }
+    double Uout[4] = {rinf, rinf * uinf, rinf * vinf, pinf / G1 + 0.5 * rinf * q2inf};
+    double ra = 0.5 * (P1.v[0] + P2.v[0]);
+    double ua = 0.5 * (P1.v[1] + P2.v[1]);
+    double va = 0.5 * (P1.v[2] + P2.v[2]);
+    double pa = 0.5 * (P1.v[3] + P2.v[3]);
+    double qa2 = sq(ua) + sq(va);
+    double aa2 = G * pa / ra;
+    double aa = sqrt(G * pa / ra);
+    double ha = aa2 / G1 + 0.5 * qa2;
+    V4 U1, U2;
+    prim2con(c, P1, U1);
+    prim2con(c, P2, U2);
+    double ct = (x2.y - x1.y), st = -(x2.x - x1.x);
+    double lent = sqrt(sq(ct) + sq(st));
+    ct = ct / lent; st = st / lent;
+    double una = ua * ct + va * st, vna = -ua * st + va * ct;
+    double lam[4] = {una, una, una + aa, una - aa}, lp[4], lm[4];
+    lp[0] = fmax(lam[0], 0.0); lp[1] = lp[0]; lp[2] = fmax(lam[2], 0.0); lp[3] = fmax(lam[3], 0.0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) lm[k] = lam[k] - lp[k];
+    double T[4][4], S[4][4];
+    double t1 = 0.5 * ra / aa, m2 = qa2 / aa2;
+    T[0][0] = 1.0; T[1][0] = ua; T[2][0] = va; T[3][0] = 0.5 * qa2;
+    T[0][1] = 0.0; T[1][1] = ra * st; T[2][1] = -ra * ct; T[3][1] = -ra * vna;
+    T[0][2] = t1; T[1][2] = t1 * (ua + aa * ct); T[2][2] = t1 * (va + aa * st); T[3][2] = t1 * (ha + aa * una);
+    T[0][3] = t1; T[1][3] = t1 * (ua - aa * ct); T[2][3] = t1 * (va - aa * st); T[3][3] = t1 * (ha - aa * una);
+    t1 = 0.5 * G1 * m2 * aa / ra;
+    double t2 = una / ra, t3 = G1 * ua / aa, t4 = G1 * va / aa, t5 = G1 / (ra * aa);
+    S[0][0] = 1.0 - 0.5 * G1 * m2; S[1][0] = vna / ra; S[2][0] = t1 - t2; S[3][0] = t1 + t2;
+    S[0][1] = t3 / aa; S[1][1] = st / ra; S[2][1] = (ct - t3) / ra; S[3][1] = -(ct + t3) / ra;
+    S[0][2] = t4 / aa; S[1][2] = -ct / ra; S[2][2] = (st - t4) / ra; S[3][2] = -(st + t4) / ra;
+    S[0][3] = -G1 / aa2; S[1][3] = 0.0; S[2][3] = t5; S[3][3] = t5;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double jp[4], jm[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            double a = 0.0, b = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                a = a + (lp[k] * T[i][k]) * S[k][j];
+                b = b + (lm[k] * T[i][k]) * S[k][j];
+            }
+            jp[j] = a; jm[j] = b;
+        }
+        double f1 = 0.0, f2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            f1 = f1 + jp[j] * U1.v[j] + jm[j] * Uout[j];
+            f2 = f2 + jp[j] * U2.v[j] + jm[j] * Uout[j];
+        }
+        r1.v[i] = 0.5 * lent * f1;
+        r2.v[i] = 0.5 * lent * f2;
+    }
+}
+
+// Sutherland viscosity, src/viscosity.f:245-258
+__device__ __forceinline__ double sutherland_viscosity(const Consts &c, double rho, double p)
+{
+    double temp = p / rho / 1.0;
+    double num = c.T_inf + c.SCONST;
+    double den = temp + c.SCONST;
+    return pow(temp / c.T_inf, 1.5) * (num / den) / c.rey;
+}
+
+// SA eddy viscosity, src/viscosity.f:261-274
+__device__ __forceinline__ double mu_turb(const Consts &c, double mu_lam, double rho, double nu)
+{
+    double chi = nu * rho / mu_lam;
+    double chi3 = pow3(chi);
+    double fv1 = chi3 / (chi3 + c.Cv11);
+    return rho * nu * fv1;
+}
+
+}  // namespace fvm

@@ -1,0 +1,244 @@
+"""Host-side mirror of fvm2d's driver (src/main.f) over the HIP library.
+
+    g = mesh.naca0012_129x33()
+    s = Solver(g, Inputs(mach_inf=0.5, aoa_deg=2.0))
+    hist, status = s.iterate(1000)          # rows: iter fres iresi fresi cl cd
+
+Names follow the reference: :func:`geometric` (src/geometric.f),
+:meth:`Solver.residu` (src/residu.f), :meth:`Solver.iterate` (the loop of
+src/main.f:42-76).  Arrays use the reference layout: prim (np, 4) rows are
+(rho, u, v, p) per vertex, connectivity 1-based.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from . import gpu
+from .gpu import FvmError, Mesh, Params
+
+DIMS = ("np", "nt", "ne", "netot", "nsp", "nfp", "nbe", "nswe1", "nswe2", "nffe1", "nffe2")
+
+
+@dataclasses.dataclass
+class Inputs:
+    """The input deck (src/read_input.f:20-66) plus options it does not hold."""
+    iflow: int = 1            # FlowType: 1 Euler, 2 laminar, 3 SA-RANS
+    mach_inf: float = 0.5
+    aoa_deg: float = 2.0
+    rey: float = 0.0
+    cfl: float = 1.0
+    iterlast: int = 0
+    maxiter: int = 1000
+    minres: float = 0.0
+    saveinterval: int = 10**9
+    iflux: int = 1            # 1 Roe, 2 KFVS, 3 HLLC
+    ilimit: int = 1
+    vortex: int = 0
+    xref: float = 0.25
+    yref: float = 0.0
+    cell_type: int = 1        # 1 median, 2 Barth
+    sa_compat: int = 1
+    hllc_rotate: int = 0
+    precision: int = 0        # 0 FP64, 1 split FP32 flux / FP64 accumulation
+    nut_factor: float = 0.1
+    istart: int = 1
+
+    def params(self) -> Params:
+        return Params(iflow=self.iflow, iflux=self.iflux, ilimit=self.ilimit, vortex=self.vortex,
+                      cell_type=self.cell_type, iterlast=self.iterlast, maxiter=self.maxiter,
+                      saveinterval=self.saveinterval, mach_inf=self.mach_inf,
+                      aoa_deg=self.aoa_deg, rey=self.rey, cfl=self.cfl, minres=self.minres,
+                      xref=self.xref, yref=self.yref, sa_compat=self.sa_compat,
+                      hllc_rotate=self.hllc_rotate, precision=self.precision, reserved=0,
+                      nut_factor=self.nut_factor)
+
+    @classmethod
+    def from_kwargs(cls, **kw) -> "Inputs":
+        names = {f.name for f in dataclasses.fields(cls)}
+        return cls(**{k: v for k, v in kw.items() if k in names})
+
+
+@dataclasses.dataclass
+class Geometry:
+    """What geometric() leaves in COMMON /dims/ and /gdata/ and its outputs."""
+    dims: dict
+    coord: np.ndarray
+    elem: np.ndarray
+    ptype: np.ndarray
+    edge: np.ndarray
+    edneigh: np.ndarray
+    iedge: np.ndarray
+    ds: np.ndarray
+    dsb: np.ndarray
+    elarea: np.ndarray
+    cvarea: np.ndarray
+    mcarea: np.ndarray
+    drmin: np.ndarray
+    wd: np.ndarray
+    wd1: np.ndarray
+    beindx: np.ndarray
+    spts: np.ndarray
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data
+
+
+def geometric(coord, elem, ptype, cell_type: int = 1, iflow: int = 1,
+              sa_compat: int = 1) -> Geometry:
+    """Dual-mesh preprocessing (src/geometric.f:2-63) through fvm_geometric."""
+    L = gpu.lib()
+    coord = np.ascontiguousarray(coord, dtype=np.float64).reshape(-1, 2)
+    elem = np.ascontiguousarray(elem, dtype=np.int32).reshape(-1, 3).copy()
+    ptype = np.ascontiguousarray(ptype, dtype=np.int32).ravel()
+    npt, nt = coord.shape[0], elem.shape[0]
+    me = 3 * nt + 3
+    edge = np.zeros((me, 2), np.int32)
+    edneigh = np.zeros((me, 2), np.int32)
+    iedge = np.zeros(me, np.int32)
+    ds = np.zeros((me, 2))
+    dsb = np.zeros((npt, 2))
+    elarea = np.zeros(nt)
+    cvarea = np.zeros(npt)
+    mcarea = np.zeros(npt)
+    drmin = np.zeros(npt)
+    wd = np.zeros(npt)
+    wd1 = np.zeros(npt)
+    beindx = np.zeros(me, np.int32)
+    spts = np.zeros(npt, np.int32)
+    dims = np.zeros(12, np.int32)
+    rc = L.fvm_geometric(npt, nt, _p(coord), _p(elem), _p(ptype), cell_type, iflow, sa_compat, me,
+                         _p(edge), _p(edneigh), _p(iedge), _p(ds), _p(dsb), _p(elarea),
+                         _p(cvarea), _p(mcarea), _p(drmin), _p(wd), _p(wd1), _p(beindx),
+                         _p(spts), _p(dims))
+    if rc != 0:
+        raise FvmError(rc, int(dims[11]), "fvm_geometric")
+    d = dict(zip(DIMS, (int(v) for v in dims[:11])))
+    E = d["netot"]
+    return Geometry(dims=d, coord=coord, elem=elem, ptype=ptype, edge=edge[:E].copy(),
+                    edneigh=edneigh[:E].copy(), iedge=iedge[:d["ne"]].copy(), ds=ds[:E].copy(),
+                    dsb=dsb, elarea=elarea, cvarea=cvarea, mcarea=mcarea, drmin=drmin, wd=wd,
+                    wd1=wd1[:d["nsp"]].copy(), beindx=beindx[:d["nbe"]].copy(),
+                    spts=spts[:d["nsp"]].copy())
+
+
+def mesh_struct(g: Geometry, keep: list) -> Mesh:
+    """fvm_mesh pointing into g's arrays (``keep`` holds references)."""
+    arrs = [g.coord, g.elem, g.edge, g.iedge, g.beindx, g.ptype, g.spts, g.ds, g.elarea,
+            g.cvarea, g.mcarea, g.drmin, g.wd]
+    keep.extend(arrs)
+    d = g.dims
+    return Mesh(np=d["np"], nt=d["nt"], ne=d["ne"], netot=d["netot"], nsp=d["nsp"],
+                nswe1=d["nswe1"], nswe2=d["nswe2"], nffe1=d["nffe1"], nffe2=d["nffe2"],
+                reserved=0, coord=_p(g.coord), elem=_p(g.elem), edge=_p(g.edge),
+                iedge=_p(g.iedge), beindx=_p(g.beindx), ptype=_p(g.ptype), spts=_p(g.spts),
+                ds=_p(g.ds), elarea=_p(g.elarea), cvarea=_p(g.cvarea), mcarea=_p(g.mcarea),
+                drmin=_p(g.drmin), wd=_p(g.wd))
+
+
+class Solver:
+    """One steady solve resident on the GPU (one fvm_ctx)."""
+
+    def __init__(self, grid, inputs: Inputs | None = None, geometry: Geometry | None = None):
+        self.inputs = inputs or Inputs()
+        self.L = gpu.lib()
+        if geometry is None:
+            geometry = geometric(grid.coord, grid.elem, grid.ptype, self.inputs.cell_type,
+                                 self.inputs.iflow, self.inputs.sa_compat)
+        self.geo = geometry
+        self.np = geometry.dims["np"]
+        keep: list = []
+        m = mesh_struct(geometry, keep)
+        par = self.inputs.params()
+        h = ctypes.c_void_p()
+        rc = self.L.fvm_gpu_create(ctypes.byref(par), ctypes.byref(m), ctypes.byref(h))
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_gpu_create")
+        self.h = h
+        self.status = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fvm_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # state ------------------------------------------------------------------
+    def set_state(self, prim, nut=None):
+        prim = np.ascontiguousarray(prim, dtype=np.float64).reshape(-1)
+        nutp = None
+        if nut is not None:
+            nut = np.ascontiguousarray(nut, dtype=np.float64).reshape(-1)
+            nutp = _p(nut)
+        rc = self.L.fvm_gpu_set_state(self.h, _p(prim), nutp)
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_gpu_set_state")
+        self.status = 0
+
+    def set_cl(self, cl: float):
+        """COMMON /liftdrag/ cl, read by the farfield vortex correction."""
+        rc = self.L.fvm_gpu_set_cl(self.h, float(cl))
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_gpu_set_cl")
+
+    def get_state(self) -> dict:
+        n = self.np
+        prim = np.zeros(4 * n); nut = np.zeros(n); qx = np.zeros(4 * n); qy = np.zeros(4 * n)
+        mu = np.zeros(n)
+        rc = self.L.fvm_gpu_get_state(self.h, _p(prim), _p(nut), _p(qx), _p(qy), _p(mu))
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_gpu_get_state")
+        return dict(prim=prim, nut=nut, qx=qx, qy=qy, mu=mu)
+
+    # hot path ---------------------------------------------------------------
+    def residu(self, prim, nut=None) -> np.ndarray:
+        """res(4,np) of src/residu.f on the given state (flattened)."""
+        prim = np.ascontiguousarray(prim, dtype=np.float64).reshape(-1)
+        res = np.zeros(4 * self.np)
+        nutp = None
+        if nut is not None:
+            nut = np.ascontiguousarray(nut, dtype=np.float64).reshape(-1)
+            nutp = _p(nut)
+        rc = self.L.fvm_gpu_residu(self.h, _p(prim), nutp, _p(res))
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_gpu_residu")
+        return res
+
+    def iterate(self, n: int):
+        """Up to n iterations; returns (hist rows, status).  Raises FvmError
+        on a reference STOP condition, with the rows done before it on
+        ``err.hist``."""
+        hist = np.zeros((max(n, 1), 6))
+        nd = ctypes.c_int(0)
+        ei = ctypes.c_int(0)
+        rc = self.L.fvm_gpu_iterate(self.h, n, _p(hist), ctypes.byref(nd), ctypes.byref(ei))
+        hist = hist[: nd.value]
+        self.status = rc
+        if rc < 0:
+            err = FvmError(rc, ei.value, "fvm_gpu_iterate")
+            err.hist = hist
+            raise err
+        return hist, rc
+
+    def time_kernels(self, n: int) -> dict:
+        ms = np.zeros(16)
+        names = (ctypes.c_char_p * 16)()
+        k = self.L.fvm_gpu_time_kernels(self.h, n, _p(ms), names, 16)
+        return {names[i].decode(): float(ms[i]) for i in range(min(k, 16))}
+
+    def footprint(self) -> int:
+        return int(self.L.fvm_gpu_footprint(self.h))

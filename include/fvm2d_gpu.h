@@ -1,0 +1,147 @@
+/*
+ * fvm2d_gpu.h -- C-ABI of the MI355X (gfx950) implementation of fvm2d's
+ * cell-vertex residual and RK3 update.
+ *
+ * Conventions (those of the reference's Fortran, so that a Fortran host can
+ * bind every entry point with iso_c_binding and pass its own arrays):
+ *   - arrays are column-major with the component first: prim(4,np) is
+ *     prim[4*(p-1) + k-1]; coord(2,np); elem(3,nt); edge(2,netot); ds(2,netot)
+ *   - connectivity is 1-based
+ *   - every pointer is borrowed for the duration of the call only
+ *   - every entry point returns an fvm_status (0 = OK)
+ * The reference has no plugin API: the interfaces below replace subroutine
+ * calls of its driver loop (src/main.f:42-76) and of geometric()
+ * (src/main.f:25-26).  The file:line each one replaces is given with it.
+ */
+#ifndef FVM2D_GPU_H
+#define FVM2D_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FVM_OK = 0,
+    FVM_CONVERGED = 1,          /* fres <= MinRes (src/main.f:42)                  */
+    FVM_ITER_LIMIT = 2,         /* iter reached LastIter+MaxIter (src/main.f:42,70) */
+    FVM_E_DEGENERATE = -1,      /* src/tri_orient.f:28-31 ("triangle is degenerate") */
+    FVM_E_AREA = -2,            /* src/geom_stat.f:18-21,34-37, src/areas.f:145-150  */
+    FVM_E_ORPHAN_EDGE = -3,     /* src/flength.f:20-27                              */
+    FVM_E_NEG_SOUND = -4,       /* src/roe_flux.f:60-65 ("Sonic speed is negative") */
+    FVM_E_NONPOSITIVE = -5,     /* src/check_positivity.f:13-24                     */
+    FVM_E_ZERO_RES1 = -6,       /* src/conres.f:28                                  */
+    FVM_E_INPUT = -7,           /* src/read_input.f:72-110, src/residu.f:67-70      */
+    FVM_E_SIZE = -8,            /* a caller-sized array is too small                */
+    FVM_E_HIP = -10,            /* a HIP runtime call failed                        */
+    FVM_E_NOMEM = -11           /* device allocation failed                         */
+} fvm_status;
+
+/* Values of the COMMON blocks the hot path reads (src/param.h:56-142,
+ * src/common.h:10-19), set from the input deck (src/read_input.f:20-66). */
+typedef struct {
+    int iflow;          /* 1 inviscid, 2 laminar, 3 turbulent (FlowType)        */
+    int iflux;          /* 1 Roe, 2 KFVS, 3 HLLC (Flux)                          */
+    int ilimit;         /* 0/1 van Albada limiter (Limiter)                      */
+    int vortex;         /* farfield point-vortex correction (FarBC)              */
+    int cell_type;      /* 1 median, 2 Barth (CellType)                          */
+    int iterlast;       /* LastIter                                              */
+    int maxiter;        /* MaxIter                                               */
+    int saveinterval;   /* SaveInt: with iter >= MaxIter, stops at a multiple    */
+    double mach_inf, aoa_deg, rey, cfl, minres, xref, yref;
+    /* options the deck does not hold */
+    int sa_compat;      /* 1: nu_tilde gradient from vertices 1..3 as in
+                           src/sa_model.f:57-62 (and wall_dist.f:46's dy3);
+                           0: the vertices of the triangle                      */
+    int hllc_rotate;    /* 0: HLLC flux left in the edge frame as in
+                           src/hllc_flux.f:105-154; 1: rotated to x-y          */
+    int precision;      /* 0 FP64 throughout; 1 split: FP32 reconstruction and
+                           flux, FP64 geometry, accumulation, update, SA, norms */
+    int reserved;
+    double nut_factor;  /* initial nu_tilde = nut_factor*mu_l/rho
+                           (src/initialize.f:190 uses 0.1)                       */
+} fvm_params;
+
+/* Mesh arrays as geometric() leaves them (src/geometric.f:15-60). */
+typedef struct {
+    int np, nt, ne, netot, nsp, nswe1, nswe2, nffe1, nffe2;
+    int reserved;
+    const double *coord;   /* (2,np)                                           */
+    const int *elem;       /* (3,nt), counter-clockwise                        */
+    const int *edge;       /* (2,netot), interior triangle on the left         */
+    const int *iedge;      /* (ne) live edges (src/flength.f:84-105)           */
+    const int *beindx;     /* (nffe2) solid then farfield edges                */
+    const int *ptype;      /* (np) 0 interior, 3 solid, 5 farfield             */
+    const int *spts;       /* (nsp) solid points                               */
+    const double *ds;      /* (2,netot) dual-face normals                      */
+    const double *elarea;  /* (nt)                                             */
+    const double *cvarea;  /* (np)                                             */
+    const double *mcarea;  /* (np) median-cell areas (gradient scaling)        */
+    const double *drmin;   /* (np) time-step length                            */
+    const double *wd;      /* (np) wall distance (turbulent only, else NULL)   */
+} fvm_mesh;
+
+typedef struct fvm_ctx fvm_ctx;
+
+/* Replaces geometric(coord,elarea,cvarea,ds,dsb,mcarea,wd,drmin)
+ * (src/geometric.f:2-63) and read_grid's point classification
+ * (src/read_grid.f:175-207).  Host code.  elem is re-oriented in place
+ * (src/tri_orient.f).  Edge arrays hold max_edges columns (3*nt suffices).
+ * wd/wd1 are written only for iflow == 3.
+ * dims: np nt ne netot nsp nfp nbe nswe1 nswe2 nffe1 nffe2 err_index. */
+int fvm_geometric(int np, int nt, const double *coord, int *elem, const int *ptype,
+                  int cell_type, int iflow, int sa_compat, int max_edges,
+                  int *edge, int *edneigh, int *iedge, double *ds, double *dsb,
+                  double *elarea, double *cvarea, double *mcarea, double *drmin,
+                  double *wd, double *wd1, int *beindx, int *spts, int *dims);
+
+/* Uploads the mesh, builds the device layout and sets the free-stream state
+ * (src/initialize.f:14-90 with Start 1).  One context = one solve, on the
+ * current HIP device, with its own stream. */
+int fvm_gpu_create(const fvm_params *par, const fvm_mesh *mesh, fvm_ctx **out);
+int fvm_gpu_destroy(fvm_ctx *ctx);
+
+/* State in the reference layout: prim(4,np) = (rho,u,v,p), nut(np).
+ * set_state is the restart path (src/initialize.f:91-103) and also
+ * restarts the iteration count at LastIter. */
+int fvm_gpu_set_state(fvm_ctx *ctx, const double *prim, const double *nut);
+/* Any output pointer may be NULL.  qx, qy, mu are those of the last residual. */
+int fvm_gpu_get_state(fvm_ctx *ctx, double *prim, double *nut, double *qx, double *qy,
+                      double *mu);
+
+/* Sets the lift coefficient in COMMON /liftdrag/ (src/param.h:113-114) that
+ * the farfield point-vortex correction reads (src/farfield_flux.f:33);
+ * fvm_gpu_iterate updates it every iteration (src/clcd.f:46). */
+int fvm_gpu_set_cl(fvm_ctx *ctx, double cl);
+
+/* One residual on the given state: replaces
+ * residu(coord,elarea,ds,prim,qx,qy,mu,mul,nut,mcarea,res) (src/residu.f:2-3).
+ * res(4,np) in the reference's summation order (bitwise for the FP64 Roe and
+ * HLLC paths).  Leaves the iteration state untouched. */
+int fvm_gpu_residu(fvm_ctx *ctx, const double *prim, const double *nut, double *res);
+
+/* Up to n iterations of the loop body src/main.f:44-63 (save_old, time_step,
+ * 3 x [residu, update], sa_model, check_positivity, clcd, conres), replayed
+ * from HIP graphs.  hist (6 x n doubles, may be NULL) receives one row
+ * (iter, fres, iresi, fresi, cl, cd) per iteration done, as main.f:63
+ * writes to RES.DAT.  *ndone = iterations done; *err_index = 1-based
+ * vertex/edge of an error.  Returns FVM_OK (more iterations allowed),
+ * FVM_CONVERGED, FVM_ITER_LIMIT or an error. */
+int fvm_gpu_iterate(fvm_ctx *ctx, int n, double *hist, int *ndone, int *err_index);
+
+/* Measurement: runs n iterations launching each kernel on its own and
+ * timing every launch with HIP events on the solver's stream.  ms[k] =
+ * average duration of kernel kind k, names[k] its name (up to nk kinds).
+ * Returns the number of kinds. */
+int fvm_gpu_time_kernels(fvm_ctx *ctx, int n, double *ms, const char **names, int nk);
+
+/* Bytes of device memory the context holds (admission by HBM footprint). */
+long long fvm_gpu_footprint(const fvm_ctx *ctx);
+/* Predicted footprint before creating a context. */
+long long fvm_gpu_footprint_estimate(int np, int nt, int netot, int iflow, int precision);
+
+const char *fvm_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,179 @@
+"""Parity of the HIP path with the reference: golden vectors written by the
+reference's own routines (tests/golden) and the oracle on the same seeded
+inputs.  Every call goes through the C-ABI (libfvm2d_gpu.so).
+
+Tolerances (the FP64 bar: residual within 1e-11 relative, CL/CD and the
+residual history within 1e-8):
+  * Roe and HLLC, Euler: no transcendental function on the path -> the
+    residual, the states and (iter, iresi, fresi) are bit-identical; fres,
+    cl and cd come from tree reductions on the GPU: 1e-12 relative;
+  * KFVS (erf, exp), laminar (pow in Sutherland), SA (pow) and the vortex
+    farfield (atan2, sin, cos, pow): device libm may differ from glibc in
+    the last bit: residual 1e-11 relative (max norm), history and states
+    1e-8.
+"""
+import numpy as np
+import pytest
+
+from fvm2d_amd import mesh
+from fvm2d_amd.gpu import FvmError
+from fvm2d_amd.solver import Inputs, Solver, geometric
+from tests.golden.cases import CASES
+from tests.helpers import case_kwargs, fixture, grid, rel_err
+
+pytestmark = pytest.mark.gpu
+
+BITWISE = {n for n, c in CASES.items()
+           if c["deck"]["FlowType"] == 1 and c["deck"]["Flux"] in (1, 3)
+           and c["deck"]["FarBC"][0] == 0}
+
+
+def solver_for(name, **over):
+    kw = case_kwargs(name, **over)
+    return Solver(grid(name), Inputs.from_kwargs(**kw))
+
+
+def check(a, b, bitwise, tol, what):
+    if bitwise:
+        np.testing.assert_array_equal(a, b, err_msg=what)
+    else:
+        assert rel_err(a, b) <= tol, (what, rel_err(a, b))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_residual_on_initial_state(name):
+    fx = fixture(name)
+    with solver_for(name) as s:
+        res = s.residu(fx["prim0"], fx["nut0"])
+    check(res, fx["res0"], name in BITWISE, 1e-11, "res0")
+
+
+@pytest.mark.parametrize("name", [n for n in CASES if "resN" in fixture(n)])
+def test_residual_on_developed_state(name):
+    fx = fixture(name)
+    with solver_for(name) as s:
+        if CASES[name]["deck"]["FarBC"][0] == 1:
+            s.set_cl(fx["hist"][-1, 4])       # cl of the last iteration, as in COMMON
+        res = s.residu(fx["primN"], fx["nutN"])
+    check(res, fx["resN"], name in BITWISE, 1e-11, "resN")
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_iterations_match_reference(name):
+    fx = fixture(name)
+    n = CASES[name]["iters"]
+    with solver_for(name) as s:
+        if CASES[name]["deck"]["Start"] == 2:
+            s.set_state(fx["prim0"], fx["nut0"])
+        if fx["stop"][0]:
+            with pytest.raises(FvmError) as ei:
+                s.iterate(n)
+            assert ei.value.code == -5 and ei.value.index == fx["stop"][0]
+            assert ei.value.hist.shape[0] == fx["hist"].shape[0]
+            return
+        hist, status = s.iterate(n)
+        st = s.get_state()
+    ref = fx["hist"]
+    assert hist.shape == ref.shape
+    np.testing.assert_array_equal(hist[:, 0], ref[:, 0])
+    bit = name in BITWISE
+    if bit:
+        np.testing.assert_array_equal(hist[:, 2], ref[:, 2])
+        np.testing.assert_array_equal(hist[:, 3], ref[:, 3])
+        np.testing.assert_array_equal(st["prim"], fx["primN"])
+    tol = 1e-12 if bit else 1e-8
+    for col in (1, 3, 4, 5):
+        err = np.abs(hist[:, col] - ref[:, col]).max() / max(np.abs(ref[:, col]).max(), 1e-300)
+        assert err <= tol, (col, err)
+    assert rel_err(st["prim"], fx["primN"]) <= (0 if bit else 1e-8)
+    if CASES[name]["deck"]["FlowType"] == 3:
+        assert rel_err(st["nut"], fx["nutN"]) <= 1e-8
+
+
+def test_bitwise_run_to_run():
+    name = "euler_roe_transonic_65x17"
+    states = []
+    for _ in range(2):
+        with solver_for(name) as s:
+            h, _ = s.iterate(80)
+            states.append((h, s.get_state()["prim"]))
+    np.testing.assert_array_equal(states[0][0], states[1][0])
+    np.testing.assert_array_equal(states[0][1], states[1][1])
+
+
+def test_chunked_iterate_equals_one_call():
+    """Graph replays of 8 iterations + single iterations, in several calls."""
+    name = "euler_roe_nolim_65x17"
+    with solver_for(name) as a:
+        ha, _ = a.iterate(45)
+        pa = a.get_state()["prim"]
+    with solver_for(name) as b:
+        parts = [b.iterate(k)[0] for k in (1, 7, 8, 13, 16)]
+        pb = b.get_state()["prim"]
+    np.testing.assert_array_equal(np.concatenate(parts), ha)
+    np.testing.assert_array_equal(pa, pb)
+
+
+def test_negative_sound_speed_stops_like_reference():
+    """A state with a negative pressure: the Roe average fails on the same
+    edge in the oracle and on the GPU (src/roe_flux.f:60-65)."""
+    from oracle.oracle import Oracle
+    name = "euler_roe_nolim_65x17"
+    g = grid(name)
+    o = Oracle(g.coord, g.elem, g.ptype, **case_kwargs(name))
+    prim = o.get("prim").copy()
+    prim[4 * 600 + 3] = -50.0
+    o.set("prim", prim)
+    _, st = o.iterate(1)
+    assert st == -4
+    with solver_for(name) as s:
+        s.set_state(prim)
+        with pytest.raises(FvmError) as ei:
+            s.iterate(3)
+    assert ei.value.code == -4
+    assert ei.value.index == o.error_index()
+
+
+def test_convergence_stop_and_no_work_after():
+    name = "euler_roe_transonic_65x17"
+    fx = fixture(name)
+    target = fx["hist"][40, 1]          # fres reached at iteration 40
+    with solver_for(name, minres=float(target)) as s:
+        h, status = s.iterate(100)
+        assert status == 1 and h.shape[0] == 41
+        p = s.get_state()["prim"]
+        h2, status2 = s.iterate(10)
+        assert h2.shape[0] == 0 and status2 == 1
+        np.testing.assert_array_equal(s.get_state()["prim"], p)
+
+
+def test_workload_grid_513x129_matches_oracle():
+    """The 66k-vertex transonic workload grid: 12 iterations, bit-identical
+    states, against the oracle on the same inputs."""
+    from oracle.oracle import Oracle
+    g = mesh.naca0012_513x129()
+    kw = dict(iflow=1, iflux=1, ilimit=1, mach_inf=0.8, aoa_deg=1.25, cfl=1.0, maxiter=2000)
+    o = Oracle(g.coord, g.elem, g.ptype, **kw)
+    ho, _ = o.iterate(12)
+    with Solver(g, Inputs.from_kwargs(**kw)) as s:
+        hg, _ = s.iterate(12)
+        st = s.get_state()
+    np.testing.assert_array_equal(st["prim"], o.get("prim"))
+    np.testing.assert_array_equal(hg[:, [0, 2, 3]], ho[:, [0, 2, 3]])
+    assert np.abs(hg[:, 4:] - ho[:, 4:]).max() < 1e-12
+
+
+@pytest.mark.parametrize("iflux", [2, 3])
+def test_workload_grid_other_fluxes_match_oracle(iflux):
+    from oracle.oracle import Oracle
+    g = mesh.naca0012_513x129()
+    kw = dict(iflow=1, iflux=iflux, ilimit=1, mach_inf=0.8, aoa_deg=1.25, cfl=1.0, maxiter=2000)
+    o = Oracle(g.coord, g.elem, g.ptype, **kw)
+    prim0 = o.get("prim")
+    res_o = o.residu()
+    with Solver(g, Inputs.from_kwargs(**kw)) as s:
+        res_g = s.residu(prim0)
+    if iflux == 3:
+        np.testing.assert_array_equal(res_g, res_o)
+    else:
+        assert rel_err(res_g, res_o) < 1e-12
