@@ -34,8 +34,9 @@ def algorithmic_bytes(kernel: str, npt: int, nt: int, ne: int) -> float | None:
       edge-flux pass F = 24 ne + (16 + 32 + 64 + 32) np
       update pass    U = (32 + 32 + 32 + 8 + 8 + 32) np
       save_old + time_step = (64 + 48) np"""
-    return {"k_grad": 12 * nt + 120 * npt, "k_flux": 24 * ne + 144 * npt,
-            "k_update": 144 * npt, "k_tstep": 112 * npt}.get(kernel)
+    G, F, U = 12 * nt + 120 * npt, 24 * ne + 144 * npt, 144 * npt
+    return {"k_grad": G, "k_grad2": G, "k_flux": F, "k_update": U, "k_fu2": F + U,
+            "k_patch": G + F + U, "k_tstep": 112 * npt}.get(kernel)
 
 
 def traffic_from_profiles(kernel: str):
@@ -80,7 +81,22 @@ def main() -> int:
     ap.add_argument("--flux", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--variant", type=int, default=None, help="kernel layout 1 or 2 (default: tuned)")
+    ap.add_argument("--vb", type=int, default=64, help="vertices per workgroup (variant 2)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES, five times: the "
+                         "reference reading for the FETCH_SIZE/WRITE_SIZE counters")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = a.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n, "copies": 5}))
+        return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the workload is a single-GPU solve and this build "
               "has no multi-GPU path (independent solves shard one per GPU).", file=sys.stderr)
@@ -94,6 +110,8 @@ def main() -> int:
     kw = dict(iflow=1, iflux=a.flux, ilimit=1, mach_inf=0.8, aoa_deg=1.25, cfl=1.0,
               maxiter=10**9, minres=0.0)
     s = Solver(g, Inputs.from_kwargs(**kw))
+    if a.variant is not None:
+        s.tune(a.variant, a.vb, 1)
     dims = s.geo.dims
     npt, nt, ne = dims["np"], dims["nt"], dims["ne"]
 
@@ -109,7 +127,8 @@ def main() -> int:
     roofline = None
     if not a.no_kernel_timing:
         ms = s.time_kernels(20)
-        per_iter = {k: v * (3 if k in ("k_grad", "k_flux", "k_update") else 1) for k, v in ms.items()}
+        per_stage = ("k_grad", "k_flux", "k_update", "k_grad2", "k_fu2", "k_visc", "k_patch")
+        per_iter = {k: v * (3 if k in per_stage else 1) for k, v in ms.items()}
         dom = max(per_iter, key=per_iter.get)
         B = algorithmic_bytes(dom, npt, nt, ne)
         if B is not None:

@@ -42,12 +42,14 @@ def _run(cmd):
 def build(verbose: bool = False) -> str:
     os.makedirs(OUTDIR, exist_ok=True)
     obj_geom = os.path.join(OUTDIR, "geometry.o")
+    obj_part = os.path.join(OUTDIR, "partition.o")
     obj_hip = os.path.join(OUTDIR, "fvm2d_gpu.o")
     _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "geometry.cpp"), "-o", obj_geom])
+    _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "partition.cpp"), "-o", obj_part])
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "fvm2d_gpu.hip"), "-o", obj_hip])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, obj_hip, obj_geom,
-          "-pthread"])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, obj_hip, obj_geom, obj_part,
+          "-pthread", "-Wl,--no-undefined"])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -33,6 +33,8 @@
 
 #include "../../include/fvm2d_gpu.h"
 #include "kernels.h"
+#include "partition.h"
+#include "patch_kernel.h"
 
 using namespace fvm;
 
@@ -66,7 +68,17 @@ struct Dev {
     double *partial;
     DevState *S;
     double *hist;
+    // slot-parallel layout: per vertex, its (vertex, triangle) slots and its
+    // flux slots (edges in edge order, then boundary edges in beindx order)
+    const int4 *tslot;                // n1, n2, n3, 4*t + (position of the vertex)
+    const int *v2s_ptr, *sslot;       // flux slots: +-(e+1) or kBndSlot + 2*b + end
+    int vb, nblocks, xcd;             // vertices per workgroup, workgroups, XCD remap
+    const int *orig;                  // 1-based vertex number in the caller's order
+    double4 *uout;                    // far-field state of each farfield edge (conserved)
 };
+
+constexpr int kChunk = 512;           // slots staged in LDS per pass
+constexpr int kBndSlot = 1 << 30;     // boundary-slot tag
 
 // ----------------------------------------------------------------- kernels
 
@@ -203,7 +215,7 @@ __global__ void __launch_bounds__(kBlock) k_flux(Dev d, Consts c, DevState *S)
         solid_flux(x1, x2, d.prim[e.x].w, d.prim[e.y].w, r1, r2);
     } else {
         V4 p1 = ld4(d.prim, e.x), p2 = ld4(d.prim, e.y);
-        farfield_flux(c, S->cl, x1, x2, p1, p2, r1, r2);
+        farfield_flux(c, d.uout[b - d.nsw], x1, x2, p1, p2, r1, r2);
     }
     st4(d.bflux, 2 * b, r1);
     st4(d.bflux, 2 * b + 1, r2);
@@ -298,9 +310,9 @@ __global__ void __launch_bounds__(kBlock) k_update(Dev d, Consts c, DevState *S)
             double gra = q.x - po.v[0];
             double dif = fabs(gra);
             red.s = sq(gra);
-            if (dif > 0.0) { red.m = dif; red.i = v + 1; }
+            if (dif > 0.0) { red.m = dif; red.i = d.orig[v]; }
             if (!TURB && (q.x <= 0.0 || q.w <= 0.0)) {
-                atomicMin(&S->err_index, v + 1);
+                atomicMin(&S->err_index, d.orig[v]);
                 S->err_code = FVM_E_NONPOSITIVE;
             }
         }
@@ -400,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_sa(Dev d, Consts c, DevState *S)
         if (d.solid[v]) nn = 0.0;
         d.nut_new[v] = nn;
         if (p.x <= 0.0 || p.w <= 0.0 || nn < 0.0) {
-            atomicMin(&S->err_index, v + 1);
+            atomicMin(&S->err_index, d.orig[v]);
             S->err_code = FVM_E_NONPOSITIVE;
         }
     }
@@ -455,10 +467,19 @@ __global__ void __launch_bounds__(kBlock) k_final(Dev d, Consts c, DevState *S)
     Red fx = block_reduce(Red{xf, 0.0, 0});
     __syncthreads();
     Red fy = block_reduce(Red{yf, 0.0, 0});
-    if (t != 0) return;
     double qdyn = 0.5 * c.r_inf * sq(c.q_inf);
     double cl = (-sin(c.aoa) * fx.s + cos(c.aoa) * fy.s) / qdyn;
     double cd = (cos(c.aoa) * fx.s + sin(c.aoa) * fy.s) / qdyn;
+    if (c.vortex == 1 && c.mach_inf < 1.0) {
+        // far-field states of the next iteration, with this iteration's cl
+        for (int b = t; b < d.nff; b += kBlock) {
+            int2 e = d.bnode[d.nsw + b];
+            double u[4];
+            farfield_state(c, cl, d.coord[e.x], d.coord[e.y], u);
+            d.uout[b] = make_double4(u[0], u[1], u[2], u[3]);
+        }
+    }
+    if (t != 0) return;
     double fres = sqrt(tot.s / c.np);
     if (S->iter == c.iterlast) {
         S->fres1 = fres;
@@ -479,7 +500,499 @@ __global__ void __launch_bounds__(kBlock) k_final(Dev d, Consts c, DevState *S)
     }
 }
 
+
+// ------------------------------------------------- slot-parallel kernels
+//
+// A workgroup owns `vb` consecutive vertices.  All slots of those vertices
+// -- (vertex, triangle) pairs for the gradient, (vertex, edge) and
+// (vertex, boundary edge) pairs for the flux -- are spread over the
+// workgroup's threads, each result is staged in LDS, and then the thread of
+// each vertex adds its slots in slot order, which is the reference's
+// summation order.  Slots are processed in passes of kChunk.
+
+// Workgroups are handed to the 8 XCDs round robin; with `xcd` set, each XCD
+// gets a contiguous range of vertex blocks so neighbouring blocks share its L2.
+__device__ __forceinline__ int logical_block(int nb, int xcd)
+{
+    int b = blockIdx.x;
+    if (!xcd) return b;
+    int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
+
+template <bool VISC>
+__global__ void __launch_bounds__(kBlock) k_grad2(Dev d, Consts c, const DevState *S)
+{
+    if (S->done) return;
+    const int b = logical_block(d.nblocks, d.xcd);
+    const int v0 = b * d.vb, v1 = min(v0 + d.vb, d.np);
+    const int t = threadIdx.x, v = v0 + t;
+    const bool own = v < v1;
+    const int s0 = d.v2t_ptr[v0], s1 = d.v2t_ptr[v1];
+    const int ks = own ? d.v2t_ptr[v] : 0, ke = own ? d.v2t_ptr[v + 1] : 0;
+    __shared__ double4 lx[kChunk], ly[kChunk];
+    __shared__ double lv[VISC ? 3 * kChunk : 1];
+    double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
+    double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
+    for (int cs = s0; cs < s1; cs += kChunk) {
+        const int ce = min(cs + kChunk, s1);
+        for (int k = cs + t; k < ce; k += kBlock) {
+            int4 sl = d.tslot[k];
+            double2 x1 = d.coord[sl.x], x2 = d.coord[sl.y], x3 = d.coord[sl.z];
+            V4 p1 = ld4(d.prim, sl.x), p2 = ld4(d.prim, sl.y), p3 = ld4(d.prim, sl.z);
+            double bx[3] = {x2.y - x3.y, x3.y - x1.y, x1.y - x2.y};
+            double by[3] = {x3.x - x2.x, x1.x - x3.x, x2.x - x1.x};
+            double dxt[4], dyt[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                dxt[i] = p1.v[i] * bx[0] + p2.v[i] * bx[1] + p3.v[i] * bx[2];
+                dyt[i] = p1.v[i] * by[0] + p2.v[i] * by[1] + p3.v[i] * by[2];
+            }
+            lx[k - cs] = make_double4(dxt[0], dxt[1], dxt[2], dxt[3]);
+            ly[k - cs] = make_double4(dyt[0], dyt[1], dyt[2], dyt[3]);
+            if (VISC) {
+                const int tri = sl.w >> 2, m = sl.w & 3;
+                const double n2b3 = 2.0 / 3.0;
+                double e1 = p1.v[3] / p1.v[0], e2 = p2.v[3] / p2.v[0], e3 = p3.v[3] / p3.v[0];
+                double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
+                double eyt = e1 * by[0] + e2 * by[1] + e3 * by[2];
+                double um = (p1.v[1] + p2.v[1] + p3.v[1]) / 3.0;
+                double vm = (p1.v[2] + p2.v[2] + p3.v[2]) / 3.0;
+                double mut = (d.mu[sl.x] + d.mu[sl.y] + d.mu[sl.z]) / 3.0;
+                double mult = (d.mul[sl.x] + d.mul[sl.y] + d.mul[sl.z]) / 3.0;
+                double prim5t = (d.nut[sl.x] + d.nut[sl.y] + d.nut[sl.z]) / 3.0;
+                double rhot = (p1.v[0] + p2.v[0] + p3.v[0]) / 3.0;
+                double chi = prim5t * rhot / mult;
+                double chi3 = pow3(chi);
+                double fv1 = chi3 / (chi3 + c.Cv11);
+                double txx = n2b3 * mut * (2.0 * dxt[1] - dyt[2]);
+                double txy = mut * (dyt[1] + dxt[2]);
+                double tyy = n2b3 * mut * (2.0 * dyt[2] - dxt[1]);
+                double gpr = c.GAMMA * (mult / c.prandtl + rhot * prim5t * fv1 / c.prandtl_turb) / c.GAMMA1;
+                double efx = txx * um + txy * vm + gpr * ext;
+                double efy = txy * um + tyy * vm + gpr * eyt;
+                double areat = 0.25 / d.elarea[tri];
+                double bxm = bx[m], bym = by[m];
+                lv[3 * (k - cs)] = (txx * bxm + txy * bym) * areat;
+                lv[3 * (k - cs) + 1] = (txy * bxm + tyy * bym) * areat;
+                lv[3 * (k - cs) + 2] = (efx * bxm + efy * bym) * areat;
+            }
+        }
+        __syncthreads();
+        if (own) {
+            const int kb = max(ks, cs), kz = min(ke, ce);
+            for (int k = kb; k < kz; ++k) {
+                double4 a = lx[k - cs], q = ly[k - cs];
+                gx[0] = gx[0] + a.x; gx[1] = gx[1] + a.y; gx[2] = gx[2] + a.z; gx[3] = gx[3] + a.w;
+                gy[0] = gy[0] + q.x; gy[1] = gy[1] + q.y; gy[2] = gy[2] + q.z; gy[3] = gy[3] + q.w;
+                if (VISC) {
+                    rv1 = rv1 + lv[3 * (k - cs)];
+                    rv2 = rv2 + lv[3 * (k - cs) + 1];
+                    rv3 = rv3 + lv[3 * (k - cs) + 2];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!own) return;
+    double ais = d.ais[v];
+    d.qx[v] = make_double4(gx[0] * ais, gx[1] * ais, gx[2] * ais, gx[3] * ais);
+    d.qy[v] = make_double4(gy[0] * ais, gy[1] * ais, gy[2] * ais, gy[3] * ais);
+    if (VISC) d.resv[v] = make_double4(0.0, rv1, rv2, rv3);
+}
+
+// Flux of one slot: an interior edge seen from one end (sign folded in) or a
+// boundary edge's contribution to one end.  false: Roe average failed.
+template <int FLUX, int LIM>
+__device__ __forceinline__ bool slot_flux(const Dev &d, const Consts &c, double cl, int s, V4 &f,
+                                          int &bad_edge)
+{
+    if (s >= kBndSlot) {
+        const int q = s - kBndSlot, bnd = q >> 1;
+        int2 e = d.bnode[bnd];
+        double2 x1 = d.coord[e.x], x2 = d.coord[e.y];
+        V4 r1, r2;
+        if (bnd < d.nsw) {
+            solid_flux(x1, x2, d.prim[e.x].w, d.prim[e.y].w, r1, r2);
+        } else {
+            V4 p1 = ld4(d.prim, e.x), p2 = ld4(d.prim, e.y);
+            farfield_flux(c, d.uout[bnd - d.nsw], x1, x2, p1, p2, r1, r2);
+        }
+        f = (q & 1) ? r2 : r1;
+        return true;
+    }
+    const int ed = (s > 0 ? s : -s) - 1;
+    int2 e = d.enode[ed];
+    double2 ds = d.eds[ed];
+    double2 x1 = d.coord[e.x], x2 = d.coord[e.y];
+    V4 p1 = ld4(d.prim, e.x), p2 = ld4(d.prim, e.y);
+    V4 qx1 = ld4(d.qx, e.x), qy1 = ld4(d.qy, e.x), qx2 = ld4(d.qx, e.y), qy2 = ld4(d.qy, e.y);
+    V4 pl, pr;
+    recon<LIM>(c, x2.x - x1.x, x2.y - x1.y, p1, p2, qx1, qy1, qx2, qy2, pl, pr);
+    bool ok = true;
+    if (FLUX == 1) ok = roe_flux(c, ds.x, ds.y, pl, pr, f);
+    else if (FLUX == 2) kfvs_flux(c, ds.x, ds.y, pl, pr, f);
+    else if (FLUX == 3) hllc_flux<0>(c, ds.x, ds.y, pl, pr, f);
+    else hllc_flux<1>(c, ds.x, ds.y, pl, pr, f);
+    if (!ok) bad_edge = ed + 1;
+    if (s < 0) {                        // r - f == r + (-f) exactly
+        f.v[0] = -f.v[0]; f.v[1] = -f.v[1]; f.v[2] = -f.v[2]; f.v[3] = -f.v[3];
+    }
+    return ok;
+}
+
+// Flux + RK stage.  MODE 0: stage IRK; MODE 1: last stage, which also
+// writes the conres partials, checks positivity and, unless SA follows,
+// starts the next iteration (save_old + time_step); MODE 2: residual only.
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
+__global__ void __launch_bounds__(kBlock) k_fu2(Dev d, Consts c, DevState *S, int irk, double4 *pout,
+                                                double4 *res_out)
+{
+    if (S->done || S->err_code) return;
+    const int b = logical_block(d.nblocks, d.xcd);
+    const int v0 = b * d.vb, v1 = min(v0 + d.vb, d.np);
+    const int t = threadIdx.x, v = v0 + t;
+    const bool own = v < v1;
+    const int s0 = d.v2s_ptr[v0], s1 = d.v2s_ptr[v1];
+    const int ks = own ? d.v2s_ptr[v] : 0, ke = own ? d.v2s_ptr[v + 1] : 0;
+    const double cl = S->cl;
+    (void)cl;
+    __shared__ double4 lf[kChunk];
+    V4 r;
+    if (VISC && own) r = ld4(d.resv, v);
+    else r.v[0] = r.v[1] = r.v[2] = r.v[3] = 0.0;
+    int bad = INT_MAX;
+    for (int cs = s0; cs < s1; cs += kChunk) {
+        const int ce = min(cs + kChunk, s1);
+        for (int k = cs + t; k < ce; k += kBlock) {
+            V4 f;
+            int be = 0;
+            if (!slot_flux<FLUX, LIM>(d, c, cl, d.sslot[k], f, be)) bad = min(bad, be);
+            st4(lf, k - cs, f);
+        }
+        __syncthreads();
+        if (own) {
+            const int kb = max(ks, cs), kz = min(ke, ce);
+            for (int k = kb; k < kz; ++k) {
+                double4 f = lf[k - cs];
+                r.v[0] = r.v[0] + f.x; r.v[1] = r.v[1] + f.y; r.v[2] = r.v[2] + f.z; r.v[3] = r.v[3] + f.w;
+            }
+        }
+        __syncthreads();
+    }
+    if (bad != INT_MAX) {
+        atomicMin(&S->err_index, bad);
+        S->err_code = FVM_E_NEG_SOUND;
+    }
+    if (MODE == 2) {
+        if (own) st4(res_out, v, r);
+        return;
+    }
+    Red red{0.0, 0.0, 0};
+    if (own) {
+        V4 po = ld4(d.prim_old, v), p = ld4(d.prim, v), u0, u1, un;
+        prim2con(c, po, u0);
+        prim2con(c, p, u1);
+        const double a = c.airk[irk - 1], bb = c.birk[irk - 1];
+        double f = d.dt[v] / d.cvarea[v];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) un.v[j] = a * u0.v[j] + bb * (u1.v[j] - f * r.v[j]);
+        if (VISC && d.solid[v]) { un.v[1] = 0.0; un.v[2] = 0.0; }
+        double4 q;
+        q.x = un.v[0];
+        q.y = un.v[1] / un.v[0];
+        q.z = un.v[2] / un.v[0];
+        q.w = c.GAMMA1 * (un.v[3] - 0.5 * (sq(un.v[1]) + sq(un.v[2])) / un.v[0]);
+        pout[v] = q;
+        if (MODE == 1) {
+            double gra = q.x - po.v[0];
+            double dif = fabs(gra);
+            red.s = sq(gra);
+            if (dif > 0.0) { red.m = dif; red.i = d.orig[v]; }
+            if (!TURB) {
+                if (q.x <= 0.0 || q.w <= 0.0) {
+                    atomicMin(&S->err_index, d.orig[v]);
+                    S->err_code = FVM_E_NONPOSITIVE;
+                }
+                // time_step of the next iteration (src/time_step.f:13-28)
+                double qq = sqrt(sq(q.y) + sq(q.z));
+                double aa = sqrt(c.GAMMA * q.w / q.x);
+                double dr = d.drmin[v];
+                if (VISC) {
+                    double dtv = 2.0 * c.GAMMA * d.mu[v] / (q.x * c.prandtl);
+                    d.dt[v] = c.cfl * sq(dr) / (dr * (qq + aa) + dtv);
+                } else {
+                    d.dt[v] = c.cfl * dr / (qq + aa);
+                }
+            }
+        }
+    }
+    if (MODE == 1) {
+        Red rr = block_reduce(red);
+        if (t == 0) {
+            d.partial[3 * b] = rr.s;
+            d.partial[3 * b + 1] = rr.m;
+            d.partial[3 * b + 2] = (double)rr.i;
+        }
+    }
+}
+
+// ------------------------------------------------- patch stage kernel
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
+__global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, PatchArgs pa,
+                                                  int irk, const double4 *__restrict__ pin,
+                                                  double4 *__restrict__ pout, double4 *res_out)
+{
+    if (S->done || S->err_code) return;
+    extern __shared__ double4 smem4[];
+    double *smem = reinterpret_cast<double *>(smem4);
+    const int b = logical_block(pa.npatch, d.xcd);
+    const int *h = pa.hdr + kPatchHdr * b;
+    const int v0 = h[PH_V0], NP = h[PH_NP], NL = h[PH_NL], NG = h[PH_NG];
+    const int NLa = (NL + 1) & ~1;               // keeps every LDS array 16-byte aligned
+    const int NT = h[PH_NTRI], NE = h[PH_NEDGE], NB = h[PH_NBND];
+    const int *halo = pa.halo + h[PH_HALO];
+    const short4 *ltri = pa.ltri + h[PH_TRI];
+    const int *gtri = pa.gtri + h[PH_TRI];
+    const int *gptr = pa.gptr + h[PH_GPTR];
+    const short *gtl = pa.gtl + h[PH_GTL];
+    const short2 *ledge = pa.ledge + h[PH_EDGE];
+    const int *ledge_id = pa.ledge_id + h[PH_EDGE];
+    const short2 *lbnd = pa.lbnd + h[PH_BND];
+    const int *lbnd_id = pa.lbnd_id + h[PH_BND];
+    const int *pptr = pa.pptr + h[PH_PPTR];
+    const short *pslot = pa.pslot + h[PH_PSL];
+    const int t = threadIdx.x;
+
+    // LDS carve-up for this patch
+    double4 *s_prim = reinterpret_cast<double4 *>(smem);
+    double2 *s_coord = reinterpret_cast<double2 *>(s_prim + NLa);
+    double *s_mu = reinterpret_cast<double *>(s_coord + NLa);
+    double *s_mul = s_mu + (VISC ? NLa : 0);
+    double *s_nut = s_mul + (VISC ? NLa : 0);
+    double4 *s_gx = reinterpret_cast<double4 *>(s_nut + (VISC ? NLa : 0));
+    double4 *s_gy = s_gx + NG;
+    double *s_work = reinterpret_cast<double *>(s_gy + NG);
+    constexpr int TW = VISC ? 14 : 8;            // doubles per local triangle
+
+    // 1. stage state of P, H1, H2
+    for (int i = t; i < NL; i += kBlock) {
+        const int gv = i < NP ? v0 + i : halo[i - NP];
+        double4 p = pin[gv];
+        s_prim[i] = p;
+        s_coord[i] = d.coord[gv];
+        if (VISC) {
+            double ml = sutherland_viscosity(c, p.x, p.w);
+            double nu = d.nut[gv];
+            double m = TURB ? ml + mu_turb(c, ml, p.x, nu) : ml;
+            s_mul[i] = ml; s_mu[i] = m; s_nut[i] = nu;
+            if (i < NP) { d.mul[gv] = ml; d.mu[gv] = m; }
+        }
+    }
+    __syncthreads();
+
+    // 2. triangles touching G
+    for (int k = t; k < NT; k += kBlock) {
+        short4 tr = ltri[k];
+        double2 x1 = s_coord[tr.x], x2 = s_coord[tr.y], x3 = s_coord[tr.z];
+        double4 a1 = s_prim[tr.x], a2 = s_prim[tr.y], a3 = s_prim[tr.z];
+        V4 p1{{a1.x, a1.y, a1.z, a1.w}}, p2{{a2.x, a2.y, a2.z, a2.w}}, p3{{a3.x, a3.y, a3.z, a3.w}};
+        double bx[3] = {x2.y - x3.y, x3.y - x1.y, x1.y - x2.y};
+        double by[3] = {x3.x - x2.x, x1.x - x3.x, x2.x - x1.x};
+        double dxt[4], dyt[4];
+        double *w = s_work + TW * k;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            dxt[i] = p1.v[i] * bx[0] + p2.v[i] * bx[1] + p3.v[i] * bx[2];
+            dyt[i] = p1.v[i] * by[0] + p2.v[i] * by[1] + p3.v[i] * by[2];
+            w[i] = dxt[i];
+            w[4 + i] = dyt[i];
+        }
+        if (VISC) {
+            const double n2b3 = 2.0 / 3.0;
+            double e1 = p1.v[3] / p1.v[0], e2 = p2.v[3] / p2.v[0], e3 = p3.v[3] / p3.v[0];
+            double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
+            double eyt = e1 * by[0] + e2 * by[1] + e3 * by[2];
+            double um = (p1.v[1] + p2.v[1] + p3.v[1]) / 3.0;
+            double vm = (p1.v[2] + p2.v[2] + p3.v[2]) / 3.0;
+            double mut = (s_mu[tr.x] + s_mu[tr.y] + s_mu[tr.z]) / 3.0;
+            double mult = (s_mul[tr.x] + s_mul[tr.y] + s_mul[tr.z]) / 3.0;
+            double prim5t = (s_nut[tr.x] + s_nut[tr.y] + s_nut[tr.z]) / 3.0;
+            double rhot = (p1.v[0] + p2.v[0] + p3.v[0]) / 3.0;
+            double chi = prim5t * rhot / mult;
+            double chi3 = pow3(chi);
+            double fv1 = chi3 / (chi3 + c.Cv11);
+            double txx = n2b3 * mut * (2.0 * dxt[1] - dyt[2]);
+            double txy = mut * (dyt[1] + dxt[2]);
+            double tyy = n2b3 * mut * (2.0 * dyt[2] - dxt[1]);
+            double gpr = c.GAMMA * (mult / c.prandtl + rhot * prim5t * fv1 / c.prandtl_turb) / c.GAMMA1;
+            w[8] = txx; w[9] = txy; w[10] = tyy;
+            w[11] = txx * um + txy * vm + gpr * ext;
+            w[12] = txy * um + tyy * vm + gpr * eyt;
+            w[13] = 0.25 / d.elarea[gtri[k]];
+        }
+    }
+    __syncthreads();
+
+    // 3. gradients of G; viscous residual of P kept by P's own thread
+    double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
+    for (int g = t; g < NG; g += kBlock) {
+        double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
+        const int k1 = gptr[g + 1];
+        for (int k = gptr[g]; k < k1; ++k) {
+            const int e = gtl[k];
+            const double *w = s_work + TW * (e >> 2);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) { gx[i] = gx[i] + w[i]; gy[i] = gy[i] + w[4 + i]; }
+            if (VISC && g < NP) {
+                short4 tr = ltri[e >> 2];
+                const int m = e & 3;
+                double2 x1 = s_coord[tr.x], x2 = s_coord[tr.y], x3 = s_coord[tr.z];
+                double bxm = m == 0 ? x2.y - x3.y : (m == 1 ? x3.y - x1.y : x1.y - x2.y);
+                double bym = m == 0 ? x3.x - x2.x : (m == 1 ? x1.x - x3.x : x2.x - x1.x);
+                double areat = w[13];
+                rv1 = rv1 + (w[8] * bxm + w[9] * bym) * areat;
+                rv2 = rv2 + (w[9] * bxm + w[10] * bym) * areat;
+                rv3 = rv3 + (w[11] * bxm + w[12] * bym) * areat;
+            }
+        }
+        const int gv = g < NP ? v0 + g : halo[g - NP];
+        const double ais = d.ais[gv];
+        double4 qx = make_double4(gx[0] * ais, gx[1] * ais, gx[2] * ais, gx[3] * ais);
+        double4 qy = make_double4(gy[0] * ais, gy[1] * ais, gy[2] * ais, gy[3] * ais);
+        s_gx[g] = qx;
+        s_gy[g] = qy;
+        if (g < NP) { d.qx[gv] = qx; d.qy[gv] = qy; }
+    }
+    __syncthreads();
+
+    // 4. fluxes of the edges and boundary edges touching P
+    int bad = INT_MAX;
+    for (int k = t; k < NE; k += kBlock) {
+        short2 e = ledge[k];
+        const int ge = ledge_id[k];
+        double2 ds = d.eds[ge];
+        double2 x1 = s_coord[e.x], x2 = s_coord[e.y];
+        double4 a1 = s_prim[e.x], a2 = s_prim[e.y];
+        double4 g1 = s_gx[e.x], h1 = s_gy[e.x], g2 = s_gx[e.y], h2 = s_gy[e.y];
+        V4 p1{{a1.x, a1.y, a1.z, a1.w}}, p2{{a2.x, a2.y, a2.z, a2.w}};
+        V4 qx1{{g1.x, g1.y, g1.z, g1.w}}, qy1{{h1.x, h1.y, h1.z, h1.w}};
+        V4 qx2{{g2.x, g2.y, g2.z, g2.w}}, qy2{{h2.x, h2.y, h2.z, h2.w}};
+        V4 pl, pr, f;
+        recon<LIM>(c, x2.x - x1.x, x2.y - x1.y, p1, p2, qx1, qy1, qx2, qy2, pl, pr);
+        bool ok = true;
+        if (FLUX == 1) ok = roe_flux(c, ds.x, ds.y, pl, pr, f);
+        else if (FLUX == 2) kfvs_flux(c, ds.x, ds.y, pl, pr, f);
+        else if (FLUX == 3) hllc_flux<0>(c, ds.x, ds.y, pl, pr, f);
+        else hllc_flux<1>(c, ds.x, ds.y, pl, pr, f);
+        if (!ok) bad = min(bad, ge + 1);
+        reinterpret_cast<double4 *>(s_work)[k] = make_double4(f.v[0], f.v[1], f.v[2], f.v[3]);
+    }
+    for (int k = t; k < NB; k += kBlock) {
+        short2 e = lbnd[k];
+        const int gb = lbnd_id[k];
+        double2 x1 = s_coord[e.x], x2 = s_coord[e.y];
+        V4 r1, r2;
+        if (gb < d.nsw) {
+            solid_flux(x1, x2, s_prim[e.x].w, s_prim[e.y].w, r1, r2);
+        } else {
+            double4 a1 = s_prim[e.x], a2 = s_prim[e.y];
+            V4 p1{{a1.x, a1.y, a1.z, a1.w}}, p2{{a2.x, a2.y, a2.z, a2.w}};
+            farfield_flux(c, d.uout[gb - d.nsw], x1, x2, p1, p2, r1, r2);
+        }
+        double4 *fw = reinterpret_cast<double4 *>(s_work) + NE + 2 * k;
+        fw[0] = make_double4(r1.v[0], r1.v[1], r1.v[2], r1.v[3]);
+        fw[1] = make_double4(r2.v[0], r2.v[1], r2.v[2], r2.v[3]);
+    }
+    if (bad != INT_MAX) {
+        atomicMin(&S->err_index, bad);
+        S->err_code = FVM_E_NEG_SOUND;
+    }
+    __syncthreads();
+
+    // 5. residual of P in the reference's order, then the RK stage
+    Red red{0.0, 0.0, 0};
+    if (t < NP) {
+        const int v = v0 + t;
+        V4 r;
+        r.v[0] = 0.0;
+        r.v[1] = VISC ? rv1 : 0.0; r.v[2] = VISC ? rv2 : 0.0; r.v[3] = VISC ? rv3 : 0.0;
+        const int k1 = pptr[t + 1];
+        const double4 *fl = reinterpret_cast<const double4 *>(s_work);
+        for (int k = pptr[t]; k < k1; ++k) {
+            const int q = pslot[k];
+            double4 f = fl[q >> 1];
+            if (q & 1) {
+                r.v[0] = r.v[0] - f.x; r.v[1] = r.v[1] - f.y; r.v[2] = r.v[2] - f.z; r.v[3] = r.v[3] - f.w;
+            } else {
+                r.v[0] = r.v[0] + f.x; r.v[1] = r.v[1] + f.y; r.v[2] = r.v[2] + f.z; r.v[3] = r.v[3] + f.w;
+            }
+        }
+        if (MODE == 2) {
+            st4(res_out, v, r);
+        } else {
+            const double4 o = d.prim[v];             // state at the start of the iteration
+            const double4 pc = s_prim[t];
+            V4 po{{o.x, o.y, o.z, o.w}}, p{{pc.x, pc.y, pc.z, pc.w}}, u0, u1, un;
+            prim2con(c, po, u0);
+            prim2con(c, p, u1);
+            const double a = c.airk[irk - 1], bb = c.birk[irk - 1];
+            const double f = d.dt[v] / d.cvarea[v];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) un.v[j] = a * u0.v[j] + bb * (u1.v[j] - f * r.v[j]);
+            if (VISC && d.solid[v]) { un.v[1] = 0.0; un.v[2] = 0.0; }
+            double4 q;
+            q.x = un.v[0];
+            q.y = un.v[1] / un.v[0];
+            q.z = un.v[2] / un.v[0];
+            q.w = c.GAMMA1 * (un.v[3] - 0.5 * (sq(un.v[1]) + sq(un.v[2])) / un.v[0]);
+            pout[v] = q;
+            if (MODE == 1) {
+                const double gra = q.x - o.x;
+                const double dif = fabs(gra);
+                red.s = sq(gra);
+                if (dif > 0.0) { red.m = dif; red.i = d.orig[v]; }
+                if (!TURB) {
+                    if (q.x <= 0.0 || q.w <= 0.0) {
+                        atomicMin(&S->err_index, d.orig[v]);
+                        S->err_code = FVM_E_NONPOSITIVE;
+                    }
+                    // time_step of the next iteration (src/time_step.f:13-28)
+                    double qq = sqrt(sq(q.y) + sq(q.z));
+                    double aa = sqrt(c.GAMMA * q.w / q.x);
+                    double dr = d.drmin[v];
+                    if (VISC) {
+                        double dtv = 2.0 * c.GAMMA * s_mu[t] / (q.x * c.prandtl);
+                        d.dt[v] = c.cfl * sq(dr) / (dr * (qq + aa) + dtv);
+                    } else {
+                        d.dt[v] = c.cfl * dr / (qq + aa);
+                    }
+                }
+            }
+        }
+    }
+    if (MODE == 1) {
+        Red rr = block_reduce(red);
+        if (t == 0) {
+            d.partial[3 * b] = rr.s;
+            d.partial[3 * b + 1] = rr.m;
+            d.partial[3 * b + 2] = (double)rr.i;
+        }
+    }
+}
+
 __global__ void k_begin_chunk(DevState *S) { S->nhist = 0; }
+
+// far-field states of all farfield edges for a given cl
+__global__ void k_farfield_state(Dev d, Consts c, double cl)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.nff) return;
+    int2 e = d.bnode[d.nsw + b];
+    double u[4];
+    farfield_state(c, cl, d.coord[e.x], d.coord[e.y], u);
+    d.uout[b] = make_double4(u[0], u[1], u[2], u[3]);
+}
 
 }  // namespace
 
@@ -498,6 +1011,12 @@ struct fvm_ctx {
     hipGraphExec_t graph_n = nullptr, graph_1 = nullptr;
     bool graphs_ok = true;
     double4 *res_tmp = nullptr;
+    double4 *prim_bak = nullptr;
+    double *nut_bak = nullptr;
+    int variant = 3;                  // 1: edge-flux buffer; 2: slot-parallel; 3: patch stage kernel
+    double4 *pbuf[3] = {nullptr, nullptr, nullptr};   // stage buffers O, B, C
+    PatchArgs pa{};
+    std::vector<int> perm;            // device vertex -> caller's vertex (0-based)
 };
 
 namespace {
@@ -640,6 +1159,112 @@ void launch_update(fvm_ctx *x)
     else hipLaunchKernelGGL((k_update<IRK, true, LAST, true>), g, b, 0, x->stream, x->d, x->c, x->S);
 }
 
+
+// Stage kernels read the stage state `pin` and write `pout`; the buffers
+// rotate O -> B -> C -> O over the three stages, O being the state at the
+// start of the iteration (which the RK combination also reads).
+Dev stage_dev(const fvm_ctx *x, const double4 *pin)
+{
+    Dev d = x->d;
+    d.prim_old = x->pbuf[0];
+    d.prim = const_cast<double4 *>(pin);
+    return d;
+}
+
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
+void launch_fu2_t(fvm_ctx *x, DevState *S, int irk, const double4 *pin, double4 *pout, double4 *res)
+{
+    hipLaunchKernelGGL((k_fu2<FLUX, LIM, VISC, MODE, TURB>), dim3(x->d.nblocks), dim3(kBlock), 0,
+                       x->stream, stage_dev(x, pin), x->c, S, irk, pout, res);
+}
+
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
+void launch_patch_t(fvm_ctx *x, DevState *S, int irk, const double4 *pin, double4 *pout, double4 *res)
+{
+    hipLaunchKernelGGL((k_patch<FLUX, LIM, VISC, MODE, TURB>), dim3(x->pa.npatch), dim3(kBlock),
+                       x->pa.lds_bytes, x->stream, x->d, x->c, S, x->pa, irk, pin, pout, res);
+}
+
+template <template <int, int, bool, int, bool> class L, int FLUX, int LIM>
+void dispatch_fl(fvm_ctx *x, DevState *S, int irk, int mode, const double4 *pin, double4 *pout,
+                 double4 *res)
+{
+    const bool visc = x->par.iflow != 1, turb = x->par.iflow == 3;
+    if (!visc) {
+        if (mode == 0) L<FLUX, LIM, false, 0, false>::go(x, S, irk, pin, pout, res);
+        else if (mode == 1) L<FLUX, LIM, false, 1, false>::go(x, S, irk, pin, pout, res);
+        else L<FLUX, LIM, false, 2, false>::go(x, S, irk, pin, pout, res);
+    } else if (!turb) {
+        if (mode == 0) L<FLUX, LIM, true, 0, false>::go(x, S, irk, pin, pout, res);
+        else if (mode == 1) L<FLUX, LIM, true, 1, false>::go(x, S, irk, pin, pout, res);
+        else L<FLUX, LIM, true, 2, false>::go(x, S, irk, pin, pout, res);
+    } else {
+        if (mode == 0) L<FLUX, LIM, true, 0, true>::go(x, S, irk, pin, pout, res);
+        else if (mode == 1) L<FLUX, LIM, true, 1, true>::go(x, S, irk, pin, pout, res);
+        else L<FLUX, LIM, true, 2, true>::go(x, S, irk, pin, pout, res);
+    }
+}
+
+template <template <int, int, bool, int, bool> class L>
+void dispatch(fvm_ctx *x, DevState *S, int irk, int mode, const double4 *pin, double4 *pout,
+              double4 *res)
+{
+    int f = x->par.iflux == 3 && x->par.hllc_rotate ? 4 : x->par.iflux;
+    switch (f * 2 + (x->par.ilimit == 1 ? 1 : 0)) {
+    case 2: dispatch_fl<L, 1, 0>(x, S, irk, mode, pin, pout, res); break;
+    case 3: dispatch_fl<L, 1, 1>(x, S, irk, mode, pin, pout, res); break;
+    case 4: dispatch_fl<L, 2, 0>(x, S, irk, mode, pin, pout, res); break;
+    case 5: dispatch_fl<L, 2, 1>(x, S, irk, mode, pin, pout, res); break;
+    case 6: dispatch_fl<L, 3, 0>(x, S, irk, mode, pin, pout, res); break;
+    case 7: dispatch_fl<L, 3, 1>(x, S, irk, mode, pin, pout, res); break;
+    case 8: dispatch_fl<L, 4, 0>(x, S, irk, mode, pin, pout, res); break;
+    default: dispatch_fl<L, 4, 1>(x, S, irk, mode, pin, pout, res); break;
+    }
+}
+
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
+struct FuLaunch {
+    static void go(fvm_ctx *x, DevState *S, int irk, const double4 *pin, double4 *pout, double4 *res)
+    { launch_fu2_t<FLUX, LIM, VISC, MODE, TURB>(x, S, irk, pin, pout, res); }
+};
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
+struct PatchLaunch {
+    static void go(fvm_ctx *x, DevState *S, int irk, const double4 *pin, double4 *pout, double4 *res)
+    { launch_patch_t<FLUX, LIM, VISC, MODE, TURB>(x, S, irk, pin, pout, res); }
+};
+
+void launch_fu2(fvm_ctx *x, DevState *S, int irk, int mode, const double4 *pin, double4 *pout,
+                double4 *res)
+{
+    dispatch<FuLaunch>(x, S, irk, mode, pin, pout, res);
+}
+
+void launch_patch(fvm_ctx *x, DevState *S, int irk, int mode, const double4 *pin, double4 *pout,
+                  double4 *res)
+{
+    dispatch<PatchLaunch>(x, S, irk, mode, pin, pout, res);
+}
+
+void launch_visc(fvm_ctx *x, DevState *S, const double4 *pin)
+{
+    const int np = x->np;
+    if (x->par.iflow == 3) hipLaunchKernelGGL(k_visc<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, stage_dev(x, pin), x->c, S);
+    else hipLaunchKernelGGL(k_visc<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, stage_dev(x, pin), x->c, S);
+}
+
+void launch_grad2(fvm_ctx *x, DevState *S, const double4 *pin)
+{
+    if (x->par.iflow != 1) hipLaunchKernelGGL(k_grad2<true>, dim3(x->d.nblocks), dim3(kBlock), 0, x->stream, stage_dev(x, pin), x->c, S);
+    else hipLaunchKernelGGL(k_grad2<false>, dim3(x->d.nblocks), dim3(kBlock), 0, x->stream, stage_dev(x, pin), x->c, S);
+}
+
+void launch_tstep(fvm_ctx *x)
+{
+    const int np = x->np;
+    if (x->par.iflow != 1) hipLaunchKernelGGL(k_tstep<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c);
+    else hipLaunchKernelGGL(k_tstep<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c);
+}
+
 // Kernel sequence of one iteration (src/main.f:44-63).  If `timer` is set,
 // each launch is bracketed by events and its duration added to the tally.
 struct Tally { std::vector<double> ms; std::vector<int> cnt; std::vector<const char *> names; };
@@ -664,10 +1289,24 @@ void one_iteration(fvm_ctx *x, Tally *tally)
         tally->ms[k] += ms;
         tally->cnt[k] += 1;
     };
-    timed("k_tstep", [&] {
-        if (visc) hipLaunchKernelGGL(k_tstep<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c);
-        else hipLaunchKernelGGL(k_tstep<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c);
-    });
+    if (x->variant >= 2) {
+        // time_step runs at the end of the previous iteration's last stage,
+        // except when the SA step (which reads this iteration's dt) follows
+        if (turb) timed("k_tstep", [&] { launch_tstep(x); });
+        for (int irk = 1; irk <= 3; ++irk) {
+            const double4 *pin = x->pbuf[irk - 1];
+            double4 *pout = x->pbuf[irk % 3];
+            const int mode = irk == 3 ? 1 : 0;
+            if (x->variant == 3) {
+                timed("k_patch", [&] { launch_patch(x, x->S, irk, mode, pin, pout, nullptr); });
+            } else {
+                if (visc) timed("k_visc", [&] { launch_visc(x, x->S, pin); });
+                timed("k_grad2", [&] { launch_grad2(x, x->S, pin); });
+                timed("k_fu2", [&] { launch_fu2(x, x->S, irk, mode, pin, pout, nullptr); });
+            }
+        }
+    } else {
+    timed("k_tstep", [&] { launch_tstep(x); });
     for (int irk = 1; irk <= 3; ++irk) {
         if (visc) {
             timed("k_visc", [&] {
@@ -689,6 +1328,7 @@ void one_iteration(fvm_ctx *x, Tally *tally)
             else launch_update<3, true>(x);
         });
     }
+    }
     if (turb) {
         timed("k_sa", [&] {
             if (x->par.sa_compat) hipLaunchKernelGGL(k_sa<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c, x->S);
@@ -700,6 +1340,27 @@ void one_iteration(fvm_ctx *x, Tally *tally)
         hipLaunchKernelGGL(k_final, dim3(1), dim3(kBlock), 0, x->stream, x->d, x->c, x->S);
     });
     if (tally) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); }
+}
+
+// Applies the tuning knobs: kernel variant, vertices per workgroup, XCD
+// remap.  Re-establishes the iteration-start invariant (prim_old = prim,
+// dt = time_step(prim)) that the variant-2 last stage maintains.
+int configure(fvm_ctx *x, int variant, int vb, int xcd)
+{
+    if (variant < 1 || variant > 3) return FVM_E_INPUT;
+    if (vb < 16 || vb > kBlock) return FVM_E_INPUT;
+    x->variant = variant;
+    x->d.vb = vb;
+    x->d.nblocks = (x->np + vb - 1) / vb;
+    x->d.xcd = xcd ? 1 : 0;
+    x->d.nparts = variant == 3 ? x->pa.npatch : (variant == 2 ? x->d.nblocks : (int)nblk(x->np));
+    if (x->graph_n) (void)hipGraphExecDestroy(x->graph_n);
+    if (x->graph_1) (void)hipGraphExecDestroy(x->graph_1);
+    x->graph_n = x->graph_1 = nullptr;
+    launch_tstep(x);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return FVM_OK;
 }
 
 int build_graph(fvm_ctx *x, int iters, hipGraphExec_t *out)
@@ -725,6 +1386,9 @@ int reset_state(fvm_ctx *x)
     DevState h{};
     h.err_index = INT_MAX;
     HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
+    if (x->nff > 0)       // COMMON cl starts at zero
+        hipLaunchKernelGGL(k_farfield_state, dim3(nblk(x->nff)), dim3(kBlock), 0, x->stream, x->d, x->c, 0.0);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));
     return FVM_OK;
 }
@@ -752,10 +1416,29 @@ void drop_graphs(fvm_ctx *x)
 
 
 // prim, nut in the reference layout; mu/mul follow (src/initialize.f:187-207)
-int load_state(fvm_ctx *x, const double *prim, const double *nut)
+// caller order <-> device order (vertices renumbered by patch)
+std::vector<double> to_device(const fvm_ctx *x, const double *a, int comps)
+{
+    std::vector<double> out((size_t)x->np * comps);
+    for (int n = 0; n < x->np; ++n)
+        for (int k = 0; k < comps; ++k) out[(size_t)comps * n + k] = a[(size_t)comps * x->perm[n] + k];
+    return out;
+}
+
+void to_caller(const fvm_ctx *x, const std::vector<double> &a, int comps, double *out)
+{
+    for (int n = 0; n < x->np; ++n)
+        for (int k = 0; k < comps; ++k) out[(size_t)comps * x->perm[n] + k] = a[(size_t)comps * n + k];
+}
+
+int load_state(fvm_ctx *x, const double *prim_caller, const double *nut_caller)
 {
     const int np = x->np;
     const Consts &c = x->c;
+    std::vector<double> primv = to_device(x, prim_caller, 4), nutv;
+    if (nut_caller) nutv = to_device(x, nut_caller, 1);
+    const double *prim = primv.data();
+    const double *nut = nut_caller ? nutv.data() : nullptr;
     HIPCHK(hipMemcpyAsync(x->d.prim, prim, sizeof(double) * 4 * np, hipMemcpyHostToDevice, x->stream));
     std::vector<double> nu(np, 0.0), mul(np, 0.0), mu(np, 0.0);
     if (x->par.iflow != 1) {
@@ -777,7 +1460,372 @@ int load_state(fvm_ctx *x, const double *prim, const double *nut)
     HIPCHK(hipMemcpyAsync(x->d.mul, mul.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
     HIPCHK(hipMemcpyAsync(x->d.mu, mu.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
-    return reset_state(x);
+    int rc = reset_state(x);
+    if (rc) return rc;
+    if (x->d.nblocks > 0) {                 // iteration-start invariant
+        launch_tstep(x);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(x->stream));
+    }
+    return FVM_OK;
+}
+
+
+// ---- tables of the patch stage kernel ----
+struct PatchHost {
+    std::vector<int> hdr, halo, gtri, gptr, ledge_id, lbnd_id, pptr;
+    std::vector<short4> ltri;
+    std::vector<short> gtl, pslot;
+    std::vector<short2> ledge, lbnd;
+    int lds_max = 0;
+};
+
+// dynamic LDS per workgroup; the static LDS of the block reduction (5 KiB)
+// comes on top, within the 64 KiB a workgroup may address
+constexpr int kLdsLimit = 56 * 1024;
+
+int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<int> &aptr,
+                  const std::vector<int> &adj, const std::vector<int> &v2t_ptr,
+                  const std::vector<int> &v2t, const std::vector<int4> &tri,
+                  const std::vector<int> &v2e_ptr, const std::vector<int> &v2e,
+                  const std::vector<int2> &enode, const std::vector<int> &v2b_ptr,
+                  const std::vector<int> &v2b, const std::vector<int2> &bnode, bool visc,
+                  PatchHost &ph)
+{
+    const int ntri = (int)tri.size(), nedge = (int)enode.size(), nbnd = (int)bnode.size();
+    std::vector<int> loc(np, -1), tloc(ntri, -1), eloc(nedge, -1), bloc(nbnd, -1);
+    std::vector<int> lv, tl, el, bl;
+    const int npatch = (int)patch_ptr.size() - 1;
+    for (int k = 0; k < npatch; ++k) {
+        const int v0 = patch_ptr[k], v1 = patch_ptr[k + 1], NP = v1 - v0;
+        if (NP > kBlock) return FVM_E_SIZE;
+        lv.clear(); tl.clear(); el.clear(); bl.clear();
+        for (int v = v0; v < v1; ++v) { loc[v] = (int)lv.size(); lv.push_back(v); }
+        for (int i = 0; i < NP; ++i)                       // 1-ring
+            for (int j = aptr[lv[i]]; j < aptr[lv[i] + 1]; ++j)
+                if (loc[adj[j]] < 0) { loc[adj[j]] = (int)lv.size(); lv.push_back(adj[j]); }
+        const int NG = (int)lv.size();
+        for (int i = NP; i < NG; ++i)                      // 2-ring
+            for (int j = aptr[lv[i]]; j < aptr[lv[i] + 1]; ++j)
+                if (loc[adj[j]] < 0) { loc[adj[j]] = (int)lv.size(); lv.push_back(adj[j]); }
+        const int NL = (int)lv.size();
+        int h[kPatchHdr] = {0};
+        h[PH_V0] = v0; h[PH_NP] = NP; h[PH_NL] = NL; h[PH_NG] = NG;
+        h[PH_HALO] = (int)ph.halo.size();
+        for (int i = NP; i < NL; ++i) ph.halo.push_back(lv[i]);
+        // triangles of G, each once; per G-vertex its triangles in order
+        h[PH_TRI] = (int)ph.ltri.size();
+        h[PH_GPTR] = (int)ph.gptr.size();
+        h[PH_GTL] = (int)ph.gtl.size();
+        ph.gptr.push_back(0);
+        for (int g = 0; g < NG; ++g) {
+            const int gv = lv[g];
+            for (int j = v2t_ptr[gv]; j < v2t_ptr[gv + 1]; ++j) {
+                const int t = v2t[j];
+                const int4 n = tri[t];
+                if (tloc[t] < 0) {
+                    if (loc[n.x] < 0 || loc[n.y] < 0 || loc[n.z] < 0) return FVM_E_INPUT;
+                    tloc[t] = (int)tl.size();
+                    tl.push_back(t);
+                    ph.ltri.push_back(make_short4((short)loc[n.x], (short)loc[n.y], (short)loc[n.z], 0));
+                    ph.gtri.push_back(t);
+                }
+                const int m = (n.x == gv) ? 0 : ((n.y == gv) ? 1 : 2);
+                ph.gtl.push_back((short)(4 * tloc[t] + m));
+            }
+            ph.gptr.push_back((int)ph.gtl.size() - h[PH_GTL]);
+        }
+        h[PH_NTRI] = (int)tl.size();
+        // edges and boundary edges touching P, each once
+        h[PH_EDGE] = (int)ph.ledge.size();
+        h[PH_BND] = (int)ph.lbnd.size();
+        for (int i = 0; i < NP; ++i) {
+            const int v = lv[i];
+            for (int j = v2e_ptr[v]; j < v2e_ptr[v + 1]; ++j) {
+                const int e = std::abs(v2e[j]) - 1;
+                if (eloc[e] < 0) {
+                    eloc[e] = (int)el.size();
+                    el.push_back(e);
+                    ph.ledge.push_back(make_short2((short)loc[enode[e].x], (short)loc[enode[e].y]));
+                    ph.ledge_id.push_back(e);
+                }
+            }
+        }
+        for (int i = 0; i < NP; ++i) {
+            const int v = lv[i];
+            for (int j = v2b_ptr[v]; j < v2b_ptr[v + 1]; ++j) {
+                const int b = v2b[j] >> 1;
+                if (bloc[b] < 0) {
+                    bloc[b] = (int)bl.size();
+                    bl.push_back(b);
+                    ph.lbnd.push_back(make_short2((short)loc[bnode[b].x], (short)loc[bnode[b].y]));
+                    ph.lbnd_id.push_back(b);
+                }
+            }
+        }
+        const int NE = (int)el.size(), NB = (int)bl.size();
+        h[PH_NEDGE] = NE;
+        h[PH_NBND] = NB;
+        // per P-vertex terms: edges in edge order (+/-), then boundary edges
+        h[PH_PPTR] = (int)ph.pptr.size();
+        h[PH_PSL] = (int)ph.pslot.size();
+        ph.pptr.push_back(0);
+        for (int i = 0; i < NP; ++i) {
+            const int v = lv[i];
+            for (int j = v2e_ptr[v]; j < v2e_ptr[v + 1]; ++j) {
+                const int sgn = v2e[j];
+                ph.pslot.push_back((short)(2 * eloc[std::abs(sgn) - 1] + (sgn < 0 ? 1 : 0)));
+            }
+            for (int j = v2b_ptr[v]; j < v2b_ptr[v + 1]; ++j) {
+                const int q = v2b[j];
+                ph.pslot.push_back((short)(2 * (NE + 2 * bloc[q >> 1] + (q & 1))));
+            }
+            ph.pptr.push_back((int)ph.pslot.size() - h[PH_PSL]);
+        }
+        if (NL > 32767 || 4 * (int)tl.size() > 32767 || 2 * (NE + 2 * NB) > 32767) return FVM_E_SIZE;
+        h[PH_NGTL] = (int)ph.gtl.size() - h[PH_GTL];
+        h[PH_NPSL] = (int)ph.pslot.size() - h[PH_PSL];
+        const int NLa = (NL + 1) & ~1;
+        const int lds = NLa * 48 + (visc ? NLa * 24 : 0) + NG * 64 +
+                        std::max((int)tl.size() * (visc ? 14 : 8) * 8, (NE + 2 * NB) * 32);
+        ph.lds_max = std::max(ph.lds_max, lds);
+        for (int i = 0; i < kPatchHdr; ++i) ph.hdr.push_back(h[i]);
+        for (int v : lv) loc[v] = -1;
+        for (int t : tl) tloc[t] = -1;
+        for (int e : el) eloc[e] = -1;
+        for (int b : bl) bloc[b] = -1;
+    }
+    return ph.lds_max > kLdsLimit ? FVM_E_SIZE : FVM_OK;
+}
+
+
+// Host layout of one mesh on the device: vertices renumbered patch by patch
+// (triangle and edge numbers keep the reference's order, which fixes the
+// summation order; only storage moves), gather tables of every kernel
+// variant, and the patch tables of the stage kernel.  No HIP call.
+struct Layout {
+    std::vector<int> perm, iperm, patch_ptr, nadj_ptr, nadj;
+    std::vector<double2> coord, eds;
+    std::vector<int4> tri, tslot;
+    std::vector<int> v2t_ptr, v2t, v2e_ptr, v2e, v2b_ptr, v2b, v2s_ptr, sslot, orig;
+    std::vector<int2> enode, bnode;
+    std::vector<double> ais, cvarea, drmin;
+    std::vector<unsigned char> solid;
+    PatchHost ph;
+};
+
+int build_layout(const fvm_mesh *m, int iflow, int patch_size, Layout &L)
+{
+    const int np = m->np, nt = m->nt, ne = m->ne;
+    const int nsw = std::max(0, m->nswe2 - m->nswe1 + 1), nff = std::max(0, m->nffe2 - m->nffe1 + 1);
+    int target = patch_size > 0 ? std::min(patch_size, kBlock) : 128;
+    // Vertices are renumbered patch by patch (compact patches, contiguous
+    // ids).  Triangle and edge numbers keep the reference's order, which is
+    // what fixes the summation order; only storage moves.
+    std::vector<int> adj_ptr(np + 1, 0), adj;
+    for (int e = 0; e < m->netot; ++e) { ++adj_ptr[m->edge[2 * e]]; ++adj_ptr[m->edge[2 * e + 1]]; }
+    std::partial_sum(adj_ptr.begin(), adj_ptr.end(), adj_ptr.begin());
+    adj.resize(adj_ptr[np]);
+    {
+        std::vector<int> f(adj_ptr.begin(), adj_ptr.end() - 1);
+        for (int e = 0; e < m->netot; ++e) {
+            int a0 = m->edge[2 * e] - 1, a1 = m->edge[2 * e + 1] - 1;
+            adj[f[a0]++] = a1;
+            adj[f[a1]++] = a0;
+        }
+    }
+    fvm::Partition part = fvm::partition_vertices(np, adj_ptr, adj, target);
+    std::vector<int> &perm = part.perm;            // new -> old
+    std::vector<int> iperm(np);                    // old -> new
+    for (int n = 0; n < np; ++n) iperm[perm[n]] = n;
+    std::vector<int> nadj_ptr(np + 1, 0), nadj(adj.size());
+    for (int n = 0; n < np; ++n) {
+        int o = perm[n];
+        nadj_ptr[n + 1] = nadj_ptr[n] + (adj_ptr[o + 1] - adj_ptr[o]);
+        for (int k = adj_ptr[o]; k < adj_ptr[o + 1]; ++k) nadj[nadj_ptr[n] + k - adj_ptr[o]] = iperm[adj[k]];
+    }
+
+    // ---- host-side layout (0-based, new vertex numbers) ----
+    std::vector<double2> coord(np);
+    for (int n = 0; n < np; ++n) coord[n] = make_double2(m->coord[2 * perm[n]], m->coord[2 * perm[n] + 1]);
+    std::vector<int4> tri(nt);
+    for (int t = 0; t < nt; ++t)
+        tri[t] = make_int4(iperm[m->elem[3 * t] - 1], iperm[m->elem[3 * t + 1] - 1],
+                           iperm[m->elem[3 * t + 2] - 1], 0);
+    // triangles of each vertex, ascending
+    std::vector<int> v2t_ptr(np + 1, 0), v2t(3 * (size_t)nt);
+    for (int t = 0; t < nt; ++t) { ++v2t_ptr[tri[t].x + 1]; ++v2t_ptr[tri[t].y + 1]; ++v2t_ptr[tri[t].z + 1]; }
+    std::partial_sum(v2t_ptr.begin(), v2t_ptr.end(), v2t_ptr.begin());
+    {
+        std::vector<int> f(v2t_ptr.begin(), v2t_ptr.end() - 1);
+        for (int t = 0; t < nt; ++t) { v2t[f[tri[t].x]++] = t; v2t[f[tri[t].y]++] = t; v2t[f[tri[t].z]++] = t; }
+    }
+    // live edges in the order of the residual loop (src/residu.f:38-46)
+    std::vector<int2> enode(ne);
+    std::vector<double2> eds(ne);
+    for (int i = 0; i < ne; ++i) {
+        int ie = m->iedge[i] - 1;
+        enode[i] = make_int2(iperm[m->edge[2 * ie] - 1], iperm[m->edge[2 * ie + 1] - 1]);
+        eds[i] = make_double2(m->ds[2 * ie], m->ds[2 * ie + 1]);
+    }
+    // edges of each vertex, ascending, signed (+ first node, - second node)
+    std::vector<int> v2e_ptr(np + 1, 0), v2e(2 * (size_t)ne);
+    for (int i = 0; i < ne; ++i) { ++v2e_ptr[enode[i].x + 1]; ++v2e_ptr[enode[i].y + 1]; }
+    std::partial_sum(v2e_ptr.begin(), v2e_ptr.end(), v2e_ptr.begin());
+    {
+        std::vector<int> f(v2e_ptr.begin(), v2e_ptr.end() - 1);
+        for (int i = 0; i < ne; ++i) { v2e[f[enode[i].x]++] = i + 1; v2e[f[enode[i].y]++] = -(i + 1); }
+    }
+    // boundary edges: solid then farfield, as beindx; per-vertex entries
+    // 2*b (first node) or 2*b+1 (second node), in beindx order
+    int nb = nsw + nff;
+    std::vector<int2> bnode(std::max(nb, 1));
+    for (int b = 0; b < nb; ++b) {
+        int ie = m->beindx[m->nswe1 - 1 + b] - 1;
+        bnode[b] = make_int2(iperm[m->edge[2 * ie] - 1], iperm[m->edge[2 * ie + 1] - 1]);
+    }
+    std::vector<int> v2b_ptr(np + 1, 0), v2b(2 * (size_t)std::max(nb, 1));
+    for (int b = 0; b < nb; ++b) { ++v2b_ptr[bnode[b].x + 1]; ++v2b_ptr[bnode[b].y + 1]; }
+    std::partial_sum(v2b_ptr.begin(), v2b_ptr.end(), v2b_ptr.begin());
+    {
+        std::vector<int> f(v2b_ptr.begin(), v2b_ptr.end() - 1);
+        for (int b = 0; b < nb; ++b) { v2b[f[bnode[b].x]++] = 2 * b; v2b[f[bnode[b].y]++] = 2 * b + 1; }
+    }
+    // slot tables: (vertex, triangle) slots with the triangle's nodes and the
+    // vertex's position in it; flux slots = edge slots then boundary slots
+    std::vector<int4> tslot(3 * (size_t)nt);
+    for (int v = 0; v < np; ++v)
+        for (int k = v2t_ptr[v]; k < v2t_ptr[v + 1]; ++k) {
+            int t = v2t[k];
+            int4 n = tri[t];
+            int mm = (n.x == v) ? 0 : ((n.y == v) ? 1 : 2);
+            tslot[k] = make_int4(n.x, n.y, n.z, 4 * t + mm);
+        }
+    std::vector<int> v2s_ptr(np + 1, 0), sslot;
+    sslot.reserve(v2e.size() + v2b.size());
+    for (int v = 0; v < np; ++v) {
+        for (int k = v2e_ptr[v]; k < v2e_ptr[v + 1]; ++k) sslot.push_back(v2e[k]);
+        for (int k = v2b_ptr[v]; k < v2b_ptr[v + 1]; ++k) sslot.push_back(kBndSlot + v2b[k]);
+        v2s_ptr[v + 1] = (int)sslot.size();
+    }
+    std::vector<double> ais(np), cvarea(np), drmin(np);
+    const double n1b6 = 1.0 / 6.0;
+    for (int n = 0; n < np; ++n) {
+        ais[n] = n1b6 / m->mcarea[perm[n]];
+        cvarea[n] = m->cvarea[perm[n]];
+        drmin[n] = m->drmin[perm[n]];
+    }
+    std::vector<unsigned char> solid(np, 0);
+    for (int k = 0; k < m->nsp; ++k) solid[iperm[m->spts[k] - 1]] = 1;
+    std::vector<int> orig(np);
+    for (int n = 0; n < np; ++n) orig[n] = perm[n] + 1;
+
+
+    // patch tables; halve the patch size until a patch fits the LDS budget
+    for (;;) {
+        L.ph = PatchHost();
+        int rc = build_patches(np, part.patch_ptr, nadj_ptr, nadj, v2t_ptr, v2t, tri, v2e_ptr, v2e,
+                               enode, v2b_ptr, v2b, bnode, iflow != 1, L.ph);
+        if (rc == FVM_OK) break;
+        if (rc != FVM_E_SIZE || target <= 16) return rc;
+        target /= 2;
+        part = fvm::partition_vertices(np, adj_ptr, adj, target);
+        // renumbering changed: rebuild everything for the smaller patches
+        return build_layout(m, iflow, target, L);
+    }
+    L.perm = std::move(perm); L.iperm = std::move(iperm); L.patch_ptr = std::move(part.patch_ptr);
+    L.nadj_ptr = std::move(nadj_ptr); L.nadj = std::move(nadj);
+    L.coord = std::move(coord); L.eds = std::move(eds); L.tri = std::move(tri); L.tslot = std::move(tslot);
+    L.v2t_ptr = std::move(v2t_ptr); L.v2t = std::move(v2t); L.v2e_ptr = std::move(v2e_ptr);
+    L.v2e = std::move(v2e); L.v2b_ptr = std::move(v2b_ptr); L.v2b = std::move(v2b);
+    L.v2s_ptr = std::move(v2s_ptr); L.sslot = std::move(sslot); L.orig = std::move(orig);
+    L.enode = std::move(enode); L.bnode = std::move(bnode); L.ais = std::move(ais);
+    L.cvarea = std::move(cvarea); L.drmin = std::move(drmin); L.solid = std::move(solid);
+    return FVM_OK;
+}
+
+
+// Checks every index the kernels dereference against the array it indexes.
+// Returns 0 or the number of the first failed check.
+int validate_layout(const fvm_mesh *m, const Layout &L)
+{
+    const int np = m->np, nt = m->nt, ne = m->ne;
+    const int nb = (int)L.bnode.size();
+    std::vector<char> seen(np, 0);
+    if ((int)L.perm.size() != np) return 1;
+    for (int n = 0; n < np; ++n) {
+        int o = L.perm[n];
+        if (o < 0 || o >= np || seen[o]) return 2;
+        seen[o] = 1;
+        if (L.iperm[o] != n) return 3;
+    }
+    for (const int4 &t : L.tri)
+        if (t.x < 0 || t.x >= np || t.y < 0 || t.y >= np || t.z < 0 || t.z >= np) return 4;
+    for (const int2 &e : L.enode) if (e.x < 0 || e.x >= np || e.y < 0 || e.y >= np) return 5;
+    for (int b = 0; b < std::max(0, std::max(0, m->nffe2 - m->nswe1 + 1)); ++b)
+        if (L.bnode[b].x < 0 || L.bnode[b].x >= np || L.bnode[b].y < 0 || L.bnode[b].y >= np) return 6;
+    const PatchHost &ph = L.ph;
+    const int npatch = (int)L.patch_ptr.size() - 1;
+    if ((int)ph.hdr.size() != kPatchHdr * npatch || L.patch_ptr.back() != np) return 7;
+    const int tw = 8;
+    (void)tw;
+    for (int k = 0; k < npatch; ++k) {
+        const int *h = &ph.hdr[kPatchHdr * k];
+        const int v0 = h[PH_V0], NP = h[PH_NP], NL = h[PH_NL], NG = h[PH_NG];
+        const int NT = h[PH_NTRI], NE = h[PH_NEDGE], NB = h[PH_NBND];
+        if (h[PH_NGTL] != ph.gptr[h[PH_GPTR] + NG] || h[PH_NPSL] != ph.pptr[h[PH_PPTR] + NP]) return 25;
+        if (v0 != L.patch_ptr[k] || NP != L.patch_ptr[k + 1] - v0 || NP < 1 || NP > kBlock) return 8;
+        if (NG < NP || NL < NG || h[PH_HALO] + (NL - NP) > (int)ph.halo.size()) return 9;
+        for (int i = 0; i < NL - NP; ++i) {
+            int gv = ph.halo[h[PH_HALO] + i];
+            if (gv < 0 || gv >= np || (gv >= v0 && gv < v0 + NP)) return 10;
+        }
+        if (h[PH_TRI] + NT > (int)ph.ltri.size() || h[PH_GPTR] + NG + 1 > (int)ph.gptr.size()) return 11;
+        for (int i = 0; i < NT; ++i) {
+            short4 t = ph.ltri[h[PH_TRI] + i];
+            if (t.x < 0 || t.x >= NL || t.y < 0 || t.y >= NL || t.z < 0 || t.z >= NL) return 12;
+            if (ph.gtri[h[PH_TRI] + i] < 0 || ph.gtri[h[PH_TRI] + i] >= nt) return 13;
+        }
+        const int *gp = &ph.gptr[h[PH_GPTR]];
+        if (gp[0] != 0) return 14;
+        for (int g = 0; g < NG; ++g) {
+            if (gp[g + 1] < gp[g] || h[PH_GTL] + gp[g + 1] > (int)ph.gtl.size()) return 15;
+            int prev_t = -1;
+            for (int j = gp[g]; j < gp[g + 1]; ++j) {
+                int e = ph.gtl[h[PH_GTL] + j], lt = e >> 2, mm = e & 3;
+                if (lt < 0 || lt >= NT || mm > 2) return 16;
+                short4 t = ph.ltri[h[PH_TRI] + lt];
+                int node = mm == 0 ? t.x : (mm == 1 ? t.y : t.z);
+                if (node != g) return 17;
+                int gt = ph.gtri[h[PH_TRI] + lt];
+                if (gt <= prev_t) return 18;                 // ascending triangle order
+                prev_t = gt;
+            }
+        }
+        if (h[PH_EDGE] + NE > (int)ph.ledge.size() || h[PH_BND] + NB > (int)ph.lbnd.size()) return 19;
+        for (int i = 0; i < NE; ++i) {
+            short2 e = ph.ledge[h[PH_EDGE] + i];
+            int id = ph.ledge_id[h[PH_EDGE] + i];
+            if (e.x < 0 || e.x >= NG || e.y < 0 || e.y >= NG || id < 0 || id >= ne) return 20;
+        }
+        for (int i = 0; i < NB; ++i) {
+            short2 e = ph.lbnd[h[PH_BND] + i];
+            int id = ph.lbnd_id[h[PH_BND] + i];
+            if (e.x < 0 || e.x >= NL || e.y < 0 || e.y >= NL || id < 0 || id >= nb) return 21;
+        }
+        if (h[PH_PPTR] + NP + 1 > (int)ph.pptr.size()) return 22;
+        const int *pp = &ph.pptr[h[PH_PPTR]];
+        for (int i = 0; i < NP; ++i) {
+            const int v = v0 + i;
+            int want = (L.v2e_ptr[v + 1] - L.v2e_ptr[v]) + (L.v2b_ptr[v + 1] - L.v2b_ptr[v]);
+            if (pp[i + 1] - pp[i] != want || h[PH_PSL] + pp[i + 1] > (int)ph.pslot.size()) return 23;
+            for (int j = pp[i]; j < pp[i + 1]; ++j) {
+                int q = ph.pslot[h[PH_PSL] + j];
+                if (q < 0 || (q >> 1) >= NE + 2 * NB) return 24;
+            }
+        }
+    }
+    return 0;
 }
 
 }  // namespace
@@ -829,6 +1877,29 @@ long long fvm_gpu_footprint_estimate(int np, int nt, int netot, int iflow, int p
     return b;
 }
 
+int fvm_patch_plan(const fvm_mesh *m, int iflow, int patch_size, long long *st)
+{
+    for (int i = 0; i < 8; ++i) st[i] = 0;
+    if (m->np <= 0 || m->nt <= 0 || m->ne <= 0) return FVM_E_INPUT;
+    Layout L;
+    int rc = build_layout(m, iflow, patch_size, L);
+    if (rc) return rc;
+    const PatchHost &ph = L.ph;
+    const int npatch = (int)L.patch_ptr.size() - 1;
+    st[0] = npatch;
+    for (int k = 0; k < npatch; ++k) {
+        const int *h = &ph.hdr[kPatchHdr * k];
+        st[1] = std::max<long long>(st[1], h[PH_NP]);
+        st[2] = std::max<long long>(st[2], h[PH_NL]);
+        st[3] = std::max<long long>(st[3], h[PH_NTRI]);
+        st[4] = std::max<long long>(st[4], h[PH_NEDGE]);
+        st[6] += h[PH_NL] - h[PH_NP];
+    }
+    st[5] = ph.lds_max;
+    st[7] = validate_layout(m, L);
+    return st[7] ? FVM_E_INPUT : FVM_OK;
+}
+
 long long fvm_gpu_footprint(const fvm_ctx *x) { return x ? x->bytes : 0; }
 
 int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
@@ -849,90 +1920,69 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     set_consts(x);
     if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) { delete x; return FVM_E_HIP; }
 
-    // ---- host-side layout (0-based, vertex order of the caller) ----
-    std::vector<double2> coord(np);
-    for (int p = 0; p < np; ++p) coord[p] = make_double2(m->coord[2 * p], m->coord[2 * p + 1]);
-    std::vector<int4> tri(nt);
-    for (int t = 0; t < nt; ++t)
-        tri[t] = make_int4(m->elem[3 * t] - 1, m->elem[3 * t + 1] - 1, m->elem[3 * t + 2] - 1, 0);
-    // triangles of each vertex, ascending
-    std::vector<int> v2t_ptr(np + 1, 0), v2t(3 * (size_t)nt);
-    for (int t = 0; t < nt; ++t) { ++v2t_ptr[tri[t].x + 1]; ++v2t_ptr[tri[t].y + 1]; ++v2t_ptr[tri[t].z + 1]; }
-    std::partial_sum(v2t_ptr.begin(), v2t_ptr.end(), v2t_ptr.begin());
-    {
-        std::vector<int> f(v2t_ptr.begin(), v2t_ptr.end() - 1);
-        for (int t = 0; t < nt; ++t) { v2t[f[tri[t].x]++] = t; v2t[f[tri[t].y]++] = t; v2t[f[tri[t].z]++] = t; }
-    }
-    // live edges in the order of the residual loop (src/residu.f:38-46)
-    std::vector<int2> enode(ne);
-    std::vector<double2> eds(ne);
-    for (int i = 0; i < ne; ++i) {
-        int ie = m->iedge[i] - 1;
-        enode[i] = make_int2(m->edge[2 * ie] - 1, m->edge[2 * ie + 1] - 1);
-        eds[i] = make_double2(m->ds[2 * ie], m->ds[2 * ie + 1]);
-    }
-    // edges of each vertex, ascending, signed (+ first node, - second node)
-    std::vector<int> v2e_ptr(np + 1, 0), v2e(2 * (size_t)ne);
-    for (int i = 0; i < ne; ++i) { ++v2e_ptr[enode[i].x + 1]; ++v2e_ptr[enode[i].y + 1]; }
-    std::partial_sum(v2e_ptr.begin(), v2e_ptr.end(), v2e_ptr.begin());
-    {
-        std::vector<int> f(v2e_ptr.begin(), v2e_ptr.end() - 1);
-        for (int i = 0; i < ne; ++i) { v2e[f[enode[i].x]++] = i + 1; v2e[f[enode[i].y]++] = -(i + 1); }
-    }
-    // boundary edges: solid then farfield, as beindx; per-vertex entries
-    // 2*b (first node) or 2*b+1 (second node), in beindx order
-    int nb = x->nsw + x->nff;
-    std::vector<int2> bnode(std::max(nb, 1));
-    for (int b = 0; b < nb; ++b) {
-        int ie = m->beindx[m->nswe1 - 1 + b] - 1;
-        bnode[b] = make_int2(m->edge[2 * ie] - 1, m->edge[2 * ie + 1] - 1);
-    }
-    std::vector<int> v2b_ptr(np + 1, 0), v2b(2 * (size_t)std::max(nb, 1));
-    for (int b = 0; b < nb; ++b) { ++v2b_ptr[bnode[b].x + 1]; ++v2b_ptr[bnode[b].y + 1]; }
-    std::partial_sum(v2b_ptr.begin(), v2b_ptr.end(), v2b_ptr.begin());
-    {
-        std::vector<int> f(v2b_ptr.begin(), v2b_ptr.end() - 1);
-        for (int b = 0; b < nb; ++b) { v2b[f[bnode[b].x]++] = 2 * b; v2b[f[bnode[b].y]++] = 2 * b + 1; }
-    }
-    std::vector<double> ais(np), cvarea(m->cvarea, m->cvarea + np), drmin(m->drmin, m->drmin + np);
-    const double n1b6 = 1.0 / 6.0;
-    for (int p = 0; p < np; ++p) ais[p] = n1b6 / m->mcarea[p];
-    std::vector<unsigned char> solid(np, 0);
-    for (int k = 0; k < m->nsp; ++k) solid[m->spts[k] - 1] = 1;
-
+    Layout L;
+    rc = build_layout(m, par->iflow, par->patch_size, L);
+    if (!rc && validate_layout(m, L)) rc = FVM_E_INPUT;
+    if (rc) { fvm_gpu_destroy(x); return rc; }
+    x->perm = L.perm;
+    const std::vector<int> &iperm = L.iperm;
+    const int nb = x->nsw + x->nff;
     // ---- device buffers ----
     Dev &d = x->d;
     d.np = np; d.nt = nt; d.ne = ne; d.nsw = x->nsw; d.nff = x->nff; d.nsp = m->nsp;
     d.nparts = (int)nblk(np);
-    d.sa1 = 0; d.sa2 = std::min(1, np - 1); d.sa3 = std::min(2, np - 1);
+    d.sa1 = iperm[0]; d.sa2 = iperm[std::min(1, np - 1)]; d.sa3 = iperm[std::min(2, np - 1)];
 #define TRY(e) do { rc = (e); if (rc) { fvm_gpu_destroy(x); return rc; } } while (0)
-    TRY(upload(x, (double2 **)&d.coord, coord));
-    TRY(upload(x, (int4 **)&d.tri, tri));
-    TRY(upload(x, (int **)&d.v2t_ptr, v2t_ptr));
-    TRY(upload(x, (int **)&d.v2t, v2t));
-    TRY(upload(x, (int2 **)&d.enode, enode));
-    TRY(upload(x, (double2 **)&d.eds, eds));
-    TRY(upload(x, (int **)&d.v2e_ptr, v2e_ptr));
-    TRY(upload(x, (int **)&d.v2e, v2e));
-    TRY(upload(x, (int2 **)&d.bnode, bnode));
-    TRY(upload(x, (int **)&d.v2b_ptr, v2b_ptr));
-    TRY(upload(x, (int **)&d.v2b, v2b));
-    TRY(upload(x, (double **)&d.ais, ais));
-    TRY(upload(x, (double **)&d.cvarea, cvarea));
-    TRY(upload(x, (double **)&d.drmin, drmin));
-    TRY(upload(x, (unsigned char **)&d.solid, solid));
+    TRY(upload(x, (double2 **)&d.coord, L.coord));
+    TRY(upload(x, (int4 **)&d.tri, L.tri));
+    TRY(upload(x, (int **)&d.v2t_ptr, L.v2t_ptr));
+    TRY(upload(x, (int **)&d.v2t, L.v2t));
+    TRY(upload(x, (int2 **)&d.enode, L.enode));
+    TRY(upload(x, (double2 **)&d.eds, L.eds));
+    TRY(upload(x, (int **)&d.v2e_ptr, L.v2e_ptr));
+    TRY(upload(x, (int **)&d.v2e, L.v2e));
+    TRY(upload(x, (int2 **)&d.bnode, L.bnode));
+    TRY(upload(x, (int **)&d.v2b_ptr, L.v2b_ptr));
+    TRY(upload(x, (int **)&d.v2b, L.v2b));
+    TRY(upload(x, (int4 **)&d.tslot, L.tslot));
+    TRY(upload(x, (int **)&d.v2s_ptr, L.v2s_ptr));
+    TRY(upload(x, (int **)&d.sslot, L.sslot));
+    TRY(upload(x, (double **)&d.ais, L.ais));
+    TRY(upload(x, (double **)&d.cvarea, L.cvarea));
+    TRY(upload(x, (double **)&d.drmin, L.drmin));
+    TRY(upload(x, (unsigned char **)&d.solid, L.solid));
+    TRY(upload(x, (int **)&d.orig, L.orig));
+    TRY(upload(x, (int **)&x->pa.hdr, L.ph.hdr));
+    TRY(upload(x, (int **)&x->pa.halo, L.ph.halo));
+    TRY(upload(x, (short4 **)&x->pa.ltri, L.ph.ltri));
+    TRY(upload(x, (int **)&x->pa.gtri, L.ph.gtri));
+    TRY(upload(x, (int **)&x->pa.gptr, L.ph.gptr));
+    TRY(upload(x, (short **)&x->pa.gtl, L.ph.gtl));
+    TRY(upload(x, (short2 **)&x->pa.ledge, L.ph.ledge));
+    TRY(upload(x, (int **)&x->pa.ledge_id, L.ph.ledge_id));
+    TRY(upload(x, (short2 **)&x->pa.lbnd, L.ph.lbnd));
+    TRY(upload(x, (int **)&x->pa.lbnd_id, L.ph.lbnd_id));
+    TRY(upload(x, (int **)&x->pa.pptr, L.ph.pptr));
+    TRY(upload(x, (short **)&x->pa.pslot, L.ph.pslot));
+    x->pa.npatch = (int)L.patch_ptr.size() - 1;
+    x->pa.lds_bytes = L.ph.lds_max;
     TRY(dalloc(x, &d.prim, np));
+    TRY(dalloc(x, &x->pbuf[1], np));
+    TRY(dalloc(x, &x->pbuf[2], np));
     TRY(dalloc(x, &d.prim_old, np));
     TRY(dalloc(x, &d.qx, np));
     TRY(dalloc(x, &d.qy, np));
     TRY(dalloc(x, &d.dt, np));
     TRY(dalloc(x, &d.eflux, ne));
     TRY(dalloc(x, &d.bflux, 2 * (size_t)std::max(nb, 1)));
-    TRY(dalloc(x, &d.partial, 3 * (size_t)d.nparts));
+    TRY(dalloc(x, &d.uout, (size_t)std::max(x->nff, 1)));
+    TRY(dalloc(x, &d.partial, 3 * (size_t)(np / 16 + 2 + (int)L.patch_ptr.size())));
     TRY(dalloc(x, &d.hist, 6 * (size_t)kHistCap));
     TRY(dalloc(x, &x->S, 1));
     TRY(dalloc(x, &x->S_hook, 1));
     TRY(dalloc(x, &x->res_tmp, np));
+    TRY(dalloc(x, &x->prim_bak, np));
+    TRY(dalloc(x, &x->nut_bak, np));
     d.S = x->S;
     TRY(dalloc(x, &d.mu, np));
     TRY(dalloc(x, &d.mul, np));
@@ -943,7 +1993,8 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
         TRY(dalloc(x, &d.resv, np));
     }
     if (par->iflow == 3) {
-        std::vector<double> wd(m->wd, m->wd + np);
+        std::vector<double> wd(np);
+        for (int n = 0; n < np; ++n) wd[n] = m->wd[L.perm[n]];
         TRY(upload(x, (double **)&d.wd, wd));
         TRY(dalloc(x, &d.nut_new, np));
     }
@@ -960,7 +2011,13 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
         for (int p = 0; p < np; ++p) nut[p] = par->nut_factor * ml / c.r_inf;
         x->c.nut_inf = nut[0];
     }
+    x->d.vb = 64;
+    x->d.nblocks = (np + 63) / 64;
+    x->d.xcd = 1;
+    x->d.nparts = x->d.nblocks;
+    x->pbuf[0] = d.prim;
     rc = load_state(x, prim.data(), nut.data());
+    if (!rc) rc = configure(x, 3, 64, 1);
     if (rc) { fvm_gpu_destroy(x); return rc; }
     *out = x;
     return FVM_OK;
@@ -977,11 +2034,19 @@ int fvm_gpu_destroy(fvm_ctx *x)
     return FVM_OK;
 }
 
+int fvm_gpu_tune(fvm_ctx *x, int variant, int vertices_per_block, int xcd_remap)
+{
+    return configure(x, variant, vertices_per_block, xcd_remap);
+}
+
 int fvm_gpu_set_cl(fvm_ctx *x, double cl)
 {
     x->hS.cl = cl;
     DevState h = x->hS;
     HIPCHK(hipMemcpyAsync(&x->S->cl, &h.cl, sizeof(double), hipMemcpyHostToDevice, x->stream));
+    if (x->nff > 0)
+        hipLaunchKernelGGL(k_farfield_state, dim3(nblk(x->nff)), dim3(kBlock), 0, x->stream, x->d, x->c, cl);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));
     return FVM_OK;
 }
@@ -989,36 +2054,60 @@ int fvm_gpu_set_cl(fvm_ctx *x, double cl)
 int fvm_gpu_get_state(fvm_ctx *x, double *prim, double *nut, double *qx, double *qy, double *mu)
 {
     const size_t np = (size_t)x->np;
-    if (prim) HIPCHK(hipMemcpyAsync(prim, x->d.prim, 32 * np, hipMemcpyDeviceToHost, x->stream));
-    if (nut) HIPCHK(hipMemcpyAsync(nut, x->d.nut, 8 * np, hipMemcpyDeviceToHost, x->stream));
-    if (qx) HIPCHK(hipMemcpyAsync(qx, x->d.qx, 32 * np, hipMemcpyDeviceToHost, x->stream));
-    if (qy) HIPCHK(hipMemcpyAsync(qy, x->d.qy, 32 * np, hipMemcpyDeviceToHost, x->stream));
-    if (mu) HIPCHK(hipMemcpyAsync(mu, x->d.mu, 8 * np, hipMemcpyDeviceToHost, x->stream));
+    std::vector<double> a(4 * np), b(4 * np), q(4 * np), r(np), m(np);
+    HIPCHK(hipMemcpyAsync(a.data(), x->d.prim, 32 * np, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(r.data(), x->d.nut, 8 * np, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(b.data(), x->d.qx, 32 * np, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(q.data(), x->d.qy, 32 * np, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(m.data(), x->d.mu, 8 * np, hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
+    if (prim) to_caller(x, a, 4, prim);
+    if (nut) to_caller(x, r, 1, nut);
+    if (qx) to_caller(x, b, 4, qx);
+    if (qy) to_caller(x, q, 4, qy);
+    if (mu) to_caller(x, m, 1, mu);
     return FVM_OK;
 }
 
 int fvm_gpu_residu(fvm_ctx *x, const double *prim, const double *nut, double *res)
 {
     const int np = x->np;
-    HIPCHK(hipMemcpyAsync(x->d.prim, prim, sizeof(double) * 4 * np, hipMemcpyHostToDevice, x->stream));
-    if (nut && x->par.iflow == 3)
-        HIPCHK(hipMemcpyAsync(x->d.nut, nut, sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
+    // the solver's own state is put aside and restored afterwards
+    HIPCHK(hipMemcpyAsync(x->prim_bak, x->d.prim, sizeof(double) * 4 * np, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(x->nut_bak, x->d.nut, sizeof(double) * np, hipMemcpyDeviceToDevice, x->stream));
+    std::vector<double> primv = to_device(x, prim, 4), nutv;
+    HIPCHK(hipMemcpyAsync(x->d.prim, primv.data(), sizeof(double) * 4 * np, hipMemcpyHostToDevice, x->stream));
+    if (nut && x->par.iflow == 3) {
+        nutv = to_device(x, nut, 1);
+        HIPCHK(hipMemcpyAsync(x->d.nut, nutv.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
+    }
     // the hook keeps the iteration's error state apart, but shares cl for
     // the farfield vortex correction
     DevState h{};
     h.err_index = INT_MAX;
     h.cl = x->hS.cl;
     HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
-    residual_front(x, x->S_hook);
-    if (x->par.iflow != 1)
-        hipLaunchKernelGGL(k_res<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->res_tmp);
-    else
-        hipLaunchKernelGGL(k_res<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->res_tmp);
+    if (x->variant == 3) {
+        launch_patch(x, x->S_hook, 1, 2, x->d.prim, x->pbuf[1], x->res_tmp);
+    } else if (x->variant == 2) {
+        if (x->par.iflow != 1) launch_visc(x, x->S_hook, x->d.prim);
+        launch_grad2(x, x->S_hook, x->d.prim);
+        launch_fu2(x, x->S_hook, 1, 2, x->d.prim, x->pbuf[1], x->res_tmp);
+    } else {
+        residual_front(x, x->S_hook);
+        if (x->par.iflow != 1)
+            hipLaunchKernelGGL(k_res<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->res_tmp);
+        else
+            hipLaunchKernelGGL(k_res<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->res_tmp);
+    }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(res, x->res_tmp, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, x->stream));
+    std::vector<double> resv(4 * (size_t)np);
+    HIPCHK(hipMemcpyAsync(resv.data(), x->res_tmp, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipMemcpyAsync(&h, x->S_hook, sizeof h, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipMemcpyAsync(x->d.prim, x->prim_bak, sizeof(double) * 4 * np, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(x->d.nut, x->nut_bak, sizeof(double) * np, hipMemcpyDeviceToDevice, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
+    to_caller(x, resv, 4, res);
     if (h.err_code) return h.err_code;
     return FVM_OK;
 }

@@ -232,10 +232,13 @@ __device__ __forceinline__ void prim2con(const Consts &c, const V4 &p, V4 &u)
     u.v[3] = p.v[3] / c.GAMMA1 + 0.5 * p.v[0] * (sq(p.v[1]) + sq(p.v[2]));
 }
 
-// Steger-Warming farfield flux, src/farfield_flux.f:28-200.  r1/r2 are the
-// contributions to the two end points.
-__device__ inline void farfield_flux(const Consts &c, double cl, double2 x1, double2 x2,
-                                     const V4 &P1, const V4 &P2, V4 &r1, V4 &r2)
+// Far-field state seen by a farfield edge, src/farfield_flux.f:28-58: the
+// free stream, or with FarBC 1 (subsonic) the free stream plus a point
+// vortex of circulation 0.5*q_inf*cl at (xref, yref).  cl is the value of
+// the previous iteration (COMMON /liftdrag/), so this is evaluated once per
+// iteration and per edge, outside the stage kernels.  Conserved variables.
+__device__ inline void farfield_state(const Consts &c, double cl, double2 x1, double2 x2,
+                                      double *Uout)
 {
     const double G = c.GAMMA, G1 = c.GAMMA1;
     double uinf, vinf, q2inf, pinf, rinf;
@@ -259,7 +262,20 @@ __device__ inline void farfield_flux(const Consts &c, double cl, double2 x1, dou
         uinf = c.u_inf; vinf = c.v_inf; q2inf = sq(uinf) + sq(vinf);
         pinf = c.p_inf; rinf = c.r_inf;
     }
-    double Uout[4] = {rinf, rinf * uinf, rinf * vinf, pinf / G1 + 0.5 * rinf * q2inf};
+    Uout[0] = rinf;
+    Uout[1] = rinf * uinf;
+    Uout[2] = rinf * vinf;
+    Uout[3] = pinf / G1 + 0.5 * rinf * q2inf;
+}
+
+// Steger-Warming farfield flux, src/farfield_flux.f:61-200, with the far-field
+// state from farfield_state.  r1/r2 are the contributions to the two end
+// points.
+__device__ inline void farfield_flux(const Consts &c, const double4 uo, double2 x1, double2 x2,
+                                     const V4 &P1, const V4 &P2, V4 &r1, V4 &r2)
+{
+    const double G = c.GAMMA, G1 = c.GAMMA1;
+    const double Uout[4] = {uo.x, uo.y, uo.z, uo.w};
     double ra = 0.5 * (P1.v[0] + P2.v[0]);
     double ua = 0.5 * (P1.v[1] + P2.v[1]);
     double va = 0.5 * (P1.v[2] + P2.v[2]);
@@ -279,39 +295,46 @@ __device__ inline void farfield_flux(const Consts &c, double cl, double2 x1, dou
     lp[0] = fmax(lam[0], 0.0); lp[1] = lp[0]; lp[2] = fmax(lam[2], 0.0); lp[3] = fmax(lam[3], 0.0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) lm[k] = lam[k] - lp[k];
-    double T[4][4], S[4][4];
-    double t1 = 0.5 * ra / aa, m2 = qa2 / aa2;
-    T[0][0] = 1.0; T[1][0] = ua; T[2][0] = va; T[3][0] = 0.5 * qa2;
-    T[0][1] = 0.0; T[1][1] = ra * st; T[2][1] = -ra * ct; T[3][1] = -ra * vna;
-    T[0][2] = t1; T[1][2] = t1 * (ua + aa * ct); T[2][2] = t1 * (va + aa * st); T[3][2] = t1 * (ha + aa * una);
-    T[0][3] = t1; T[1][3] = t1 * (ua - aa * ct); T[2][3] = t1 * (va - aa * st); T[3][3] = t1 * (ha - aa * una);
-    t1 = 0.5 * G1 * m2 * aa / ra;
-    double t2 = una / ra, t3 = G1 * ua / aa, t4 = G1 * va / aa, t5 = G1 / (ra * aa);
-    S[0][0] = 1.0 - 0.5 * G1 * m2; S[1][0] = vna / ra; S[2][0] = t1 - t2; S[3][0] = t1 + t2;
-    S[0][1] = t3 / aa; S[1][1] = st / ra; S[2][1] = (ct - t3) / ra; S[3][1] = -(ct + t3) / ra;
-    S[0][2] = t4 / aa; S[1][2] = -ct / ra; S[2][2] = (st - t4) / ra; S[3][2] = -(st + t4) / ra;
-    S[0][3] = -G1 / aa2; S[1][3] = 0.0; S[2][3] = t5; S[3][3] = t5;
-#pragma unroll
+    // T (right eigenvectors) row by row and S (its inverse) column by column,
+    // built when needed: jac(i,j) = sum_k (l_k T(i,k)) S(k,j) in the
+    // reference's order, with a few live registers instead of two 4x4
+    // matrices and their products.
+    const double t1 = 0.5 * ra / aa, m2 = qa2 / aa2;
+    const double s1 = 0.5 * G1 * m2 * aa / ra, s2 = una / ra, s3 = G1 * ua / aa, s4 = G1 * va / aa,
+                 s5 = G1 / (ra * aa);
+#pragma unroll 1
     for (int i = 0; i < 4; ++i) {
-        double jp[4], jm[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            double a = 0.0, b = 0.0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                a = a + (lp[k] * T[i][k]) * S[k][j];
-                b = b + (lm[k] * T[i][k]) * S[k][j];
-            }
-            jp[j] = a; jm[j] = b;
-        }
+        double T0, T1, T2, T3;
+        if (i == 0)      { T0 = 1.0;       T1 = 0.0;        T2 = t1;                 T3 = t1; }
+        else if (i == 1) { T0 = ua;        T1 = ra * st;    T2 = t1 * (ua + aa * ct); T3 = t1 * (ua - aa * ct); }
+        else if (i == 2) { T0 = va;        T1 = -ra * ct;   T2 = t1 * (va + aa * st); T3 = t1 * (va - aa * st); }
+        else             { T0 = 0.5 * qa2; T1 = -ra * vna;  T2 = t1 * (ha + aa * una); T3 = t1 * (ha - aa * una); }
         double f1 = 0.0, f2 = 0.0;
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < 4; ++j) {
-            f1 = f1 + jp[j] * U1.v[j] + jm[j] * Uout[j];
-            f2 = f2 + jp[j] * U2.v[j] + jm[j] * Uout[j];
+            double S0, S1, S2, S3;
+            if (j == 0)      { S0 = 1.0 - 0.5 * G1 * m2; S1 = vna / ra;  S2 = s1 - s2;        S3 = s1 + s2; }
+            else if (j == 1) { S0 = s3 / aa;             S1 = st / ra;   S2 = (ct - s3) / ra; S3 = -(ct + s3) / ra; }
+            else if (j == 2) { S0 = s4 / aa;             S1 = -ct / ra;  S2 = (st - s4) / ra; S3 = -(st + s4) / ra; }
+            else             { S0 = -G1 / aa2;           S1 = 0.0;       S2 = s5;             S3 = s5; }
+            double a = 0.0, b = 0.0;
+            a = a + (lp[0] * T0) * S0; b = b + (lm[0] * T0) * S0;
+            a = a + (lp[1] * T1) * S1; b = b + (lm[1] * T1) * S1;
+            a = a + (lp[2] * T2) * S2; b = b + (lm[2] * T2) * S2;
+            a = a + (lp[3] * T3) * S3; b = b + (lm[3] * T3) * S3;
+            // select rather than index: a runtime index would put the
+            // vectors in scratch memory
+            const double u1 = j == 0 ? U1.v[0] : (j == 1 ? U1.v[1] : (j == 2 ? U1.v[2] : U1.v[3]));
+            const double u2 = j == 0 ? U2.v[0] : (j == 1 ? U2.v[1] : (j == 2 ? U2.v[2] : U2.v[3]));
+            const double uo = j == 0 ? Uout[0] : (j == 1 ? Uout[1] : (j == 2 ? Uout[2] : Uout[3]));
+            f1 = f1 + a * u1 + b * uo;
+            f2 = f2 + a * u2 + b * uo;
         }
-        r1.v[i] = 0.5 * lent * f1;
-        r2.v[i] = 0.5 * lent * f2;
+        const double g1 = 0.5 * lent * f1, g2 = 0.5 * lent * f2;
+        if (i == 0) { r1.v[0] = g1; r2.v[0] = g2; }
+        else if (i == 1) { r1.v[1] = g1; r2.v[1] = g2; }
+        else if (i == 2) { r1.v[2] = g1; r2.v[2] = g2; }
+        else { r1.v[3] = g1; r2.v[3] = g2; }
     }
 }
 

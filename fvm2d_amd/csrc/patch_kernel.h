@@ -1,0 +1,46 @@
+// patch_kernel.h -- one RK stage of the residual + update in a single
+// kernel, tiled over vertex patches through LDS.
+//
+// A workgroup owns a patch P of consecutive vertices.  It loads the stage
+// state of P, its 1-ring H1 and its 2-ring H2 into LDS, then
+//   1. evaluates every triangle touching G = P + H1 once (Green-Gauss terms
+//      and, for viscous flow, the Galerkin stresses);
+//   2. sums, for each vertex of G, its triangles in ascending triangle
+//      order -> gradients of G (src/green_gauss.f);
+//   3. evaluates every edge touching P once (MUSCL + Riemann flux) and every
+//      solid/farfield edge touching P (src/residu.f:37-88);
+//   4. sums, for each vertex of P, its edge and boundary terms in the
+//      reference's order and applies the RK stage (src/update.f).
+// The result goes to a different state buffer than the one read (stage
+// buffers rotate O -> B -> C -> O), so no workgroup reads a vertex another
+// one is writing.  Halo work (H1 gradients, cut edges) is recomputed by the
+// neighbouring patches; nothing crosses workgroups within a stage.
+#pragma once
+
+namespace fvm {
+
+// per-patch header
+constexpr int kPatchHdr = 24;
+enum {
+    PH_V0 = 0, PH_NP, PH_NL, PH_NG, PH_HALO, PH_NTRI, PH_TRI, PH_GPTR, PH_GTL,
+    PH_NEDGE, PH_EDGE, PH_NBND, PH_BND, PH_PPTR, PH_PSL, PH_NGTL, PH_NPSL
+};
+
+struct PatchArgs {
+    const int *hdr;         // kPatchHdr ints per patch
+    const int *halo;        // new vertex ids of H1 then H2
+    const short4 *ltri;     // local nodes of each local triangle
+    const int *gtri;        // its triangle number (for elarea)
+    const int *gptr;        // per G-vertex: range in gtl (patch-relative)
+    const short *gtl;       // 4 * local triangle + position of the vertex in it
+    const short2 *ledge;    // local end points of each local edge
+    const int *ledge_id;    // its position in the live-edge loop
+    const short2 *lbnd;     // local end points of each local boundary edge
+    const int *lbnd_id;     // its index among the boundary edges (solid first)
+    const int *pptr;        // per P-vertex: range in pslot (patch-relative)
+    const short *pslot;     // 2 * (term index) + (1 if subtracted)
+    int npatch;
+    int lds_bytes;
+};
+
+}  // namespace fvm

@@ -26,7 +26,7 @@ class Params(ctypes.Structure):
                 ("cell_type", ci), ("iterlast", ci), ("maxiter", ci), ("saveinterval", ci),
                 ("mach_inf", cd), ("aoa_deg", cd), ("rey", cd), ("cfl", cd), ("minres", cd),
                 ("xref", cd), ("yref", cd),
-                ("sa_compat", ci), ("hllc_rotate", ci), ("precision", ci), ("reserved", ci),
+                ("sa_compat", ci), ("hllc_rotate", ci), ("precision", ci), ("patch_size", ci),
                 ("nut_factor", cd)]
 
 
@@ -70,6 +70,8 @@ def lib():
     L.fvm_gpu_destroy.argtypes = [vp]
     L.fvm_gpu_set_state.restype = ci
     L.fvm_gpu_set_state.argtypes = [vp, vp, vp]
+    L.fvm_gpu_tune.restype = ci
+    L.fvm_gpu_tune.argtypes = [vp, ci, ci, ci]
     L.fvm_gpu_set_cl.restype = ci
     L.fvm_gpu_set_cl.argtypes = [vp, cd]
     L.fvm_gpu_get_state.restype = ci
@@ -80,6 +82,8 @@ def lib():
     L.fvm_gpu_iterate.argtypes = [vp, ci, vp, ctypes.POINTER(ci), ctypes.POINTER(ci)]
     L.fvm_gpu_time_kernels.restype = ci
     L.fvm_gpu_time_kernels.argtypes = [vp, ci, vp, ctypes.POINTER(ctypes.c_char_p), ci]
+    L.fvm_patch_plan.restype = ci
+    L.fvm_patch_plan.argtypes = [ctypes.POINTER(Mesh), ci, ci, vp]
     L.fvm_gpu_footprint.restype = ctypes.c_longlong
     L.fvm_gpu_footprint.argtypes = [vp]
     L.fvm_gpu_footprint_estimate.restype = ctypes.c_longlong

@@ -45,6 +45,7 @@ class Inputs:
     precision: int = 0        # 0 FP64, 1 split FP32 flux / FP64 accumulation
     nut_factor: float = 0.1
     istart: int = 1
+    patch_size: int = 0       # vertices per patch of the stage kernel (0 = 128)
 
     def params(self) -> Params:
         return Params(iflow=self.iflow, iflux=self.iflux, ilimit=self.ilimit, vortex=self.vortex,
@@ -52,7 +53,8 @@ class Inputs:
                       saveinterval=self.saveinterval, mach_inf=self.mach_inf,
                       aoa_deg=self.aoa_deg, rey=self.rey, cfl=self.cfl, minres=self.minres,
                       xref=self.xref, yref=self.yref, sa_compat=self.sa_compat,
-                      hllc_rotate=self.hllc_rotate, precision=self.precision, reserved=0,
+                      hllc_rotate=self.hllc_rotate, precision=self.precision,
+                      patch_size=self.patch_size,
                       nut_factor=self.nut_factor)
 
     @classmethod
@@ -139,6 +141,20 @@ def mesh_struct(g: Geometry, keep: list) -> Mesh:
                 drmin=_p(g.drmin), wd=_p(g.wd))
 
 
+def patch_plan(geo: Geometry, iflow: int = 1, patch_size: int = 0) -> dict:
+    """Layout of the LDS-tiled stage kernel for a mesh (host only)."""
+    keep: list = []
+    m = mesh_struct(geo, keep)
+    st = np.zeros(8, dtype=np.int64)
+    rc = gpu.lib().fvm_patch_plan(ctypes.byref(m), iflow, patch_size, _p(st))
+    names = ("patches", "max_patch", "max_local", "max_triangles", "max_edges", "lds_bytes",
+             "halo_total", "failed_check")
+    out = dict(zip(names, (int(v) for v in st)))
+    if rc != 0:
+        raise FvmError(rc, out["failed_check"], "fvm_patch_plan")
+    return out
+
+
 class Solver:
     """One steady solve resident on the GPU (one fvm_ctx)."""
 
@@ -188,6 +204,12 @@ class Solver:
         if rc != 0:
             raise FvmError(rc, 0, "fvm_gpu_set_state")
         self.status = 0
+
+    def tune(self, variant: int = 2, vertices_per_block: int = 64, xcd_remap: int = 1):
+        """Kernel variant and launch shape (see fvm_gpu_tune)."""
+        rc = self.L.fvm_gpu_tune(self.h, variant, vertices_per_block, xcd_remap)
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_gpu_tune")
 
     def set_cl(self, cl: float):
         """COMMON /liftdrag/ cl, read by the farfield vortex correction."""

@@ -56,7 +56,8 @@ typedef struct {
                            src/hllc_flux.f:105-154; 1: rotated to x-y          */
     int precision;      /* 0 FP64 throughout; 1 split: FP32 reconstruction and
                            flux, FP64 geometry, accumulation, update, SA, norms */
-    int reserved;
+    int patch_size;     /* vertices per patch of the LDS-tiled stage kernel
+                           (<= 256; 0 = default 128)                            */
     double nut_factor;  /* initial nu_tilde = nut_factor*mu_l/rho
                            (src/initialize.f:190 uses 0.1)                       */
 } fvm_params;
@@ -133,6 +134,21 @@ int fvm_gpu_iterate(fvm_ctx *ctx, int n, double *hist, int *ndone, int *err_inde
  * average duration of kernel kind k, names[k] its name (up to nk kinds).
  * Returns the number of kinds. */
 int fvm_gpu_time_kernels(fvm_ctx *ctx, int n, double *ms, const char **names, int nk);
+
+/* Tuning knobs (no counterpart in the reference; used by the autotuner):
+ * variant 1 = edge-parallel flux into a per-edge buffer + vertex gathers
+ * (three kernels per RK stage); variant 2 = slot-parallel, LDS-staged
+ * gradient and fused flux+update (two kernels per stage).  Both keep the
+ * reference's summation order.  vertices_per_block in [16, 256];
+ * xcd_remap gives each XCD a contiguous range of vertex blocks. */
+int fvm_gpu_tune(fvm_ctx *ctx, int variant, int vertices_per_block, int xcd_remap);
+
+/* Host only (no GPU needed): plans the device layout of a mesh for the
+ * LDS-tiled stage kernel and checks every index the kernels will use.
+ * stats[8]: patches, max patch vertices, max local vertices (patch and its
+ * two rings), max local triangles, max local edges, LDS bytes per
+ * workgroup, total halo vertices, failed check (0 = none). */
+int fvm_patch_plan(const fvm_mesh *mesh, int iflow, int patch_size, long long *stats);
 
 /* Bytes of device memory the context holds (admission by HBM footprint). */
 long long fvm_gpu_footprint(const fvm_ctx *ctx);
