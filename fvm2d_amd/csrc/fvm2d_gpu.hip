@@ -68,17 +68,12 @@ struct Dev {
     double *partial;
     DevState *S;
     double *hist;
-    // slot-parallel layout: per vertex, its (vertex, triangle) slots and its
-    // flux slots (edges in edge order, then boundary edges in beindx order)
-    const int4 *tslot;                // n1, n2, n3, 4*t + (position of the vertex)
-    const int *v2s_ptr, *sslot;       // flux slots: +-(e+1) or kBndSlot + 2*b + end
-    int vb, nblocks, xcd;             // vertices per workgroup, workgroups, XCD remap
+    int vb, nblocks, xcd;             // (layout 1) vertices per workgroup; XCD remap
     const int *orig;                  // 1-based vertex number in the caller's order
     double4 *uout;                    // far-field state of each farfield edge (conserved)
 };
 
-constexpr int kChunk = 512;           // slots staged in LDS per pass
-constexpr int kBndSlot = 1 << 30;     // boundary-slot tag
+
 
 // ----------------------------------------------------------------- kernels
 
@@ -501,14 +496,7 @@ __global__ void __launch_bounds__(kBlock) k_final(Dev d, Consts c, DevState *S)
 }
 
 
-// ------------------------------------------------- slot-parallel kernels
-//
-// A workgroup owns `vb` consecutive vertices.  All slots of those vertices
-// -- (vertex, triangle) pairs for the gradient, (vertex, edge) and
-// (vertex, boundary edge) pairs for the flux -- are spread over the
-// workgroup's threads, each result is staged in LDS, and then the thread of
-// each vertex adds its slots in slot order, which is the reference's
-// summation order.  Slots are processed in passes of kChunk.
+// ------------------------------------------------- launch helpers
 
 // Workgroups are handed to the 8 XCDs round robin; with `xcd` set, each XCD
 // gets a contiguous range of vertex blocks so neighbouring blocks share its L2.
@@ -520,225 +508,8 @@ __device__ __forceinline__ int logical_block(int nb, int xcd)
     return x * q + min(x, r) + k;
 }
 
-template <bool VISC>
-__global__ void __launch_bounds__(kBlock) k_grad2(Dev d, Consts c, const DevState *S)
-{
-    if (S->done) return;
-    const int b = logical_block(d.nblocks, d.xcd);
-    const int v0 = b * d.vb, v1 = min(v0 + d.vb, d.np);
-    const int t = threadIdx.x, v = v0 + t;
-    const bool own = v < v1;
-    const int s0 = d.v2t_ptr[v0], s1 = d.v2t_ptr[v1];
-    const int ks = own ? d.v2t_ptr[v] : 0, ke = own ? d.v2t_ptr[v + 1] : 0;
-    __shared__ double4 lx[kChunk], ly[kChunk];
-    __shared__ double lv[VISC ? 3 * kChunk : 1];
-    double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
-    double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
-    for (int cs = s0; cs < s1; cs += kChunk) {
-        const int ce = min(cs + kChunk, s1);
-        for (int k = cs + t; k < ce; k += kBlock) {
-            int4 sl = d.tslot[k];
-            double2 x1 = d.coord[sl.x], x2 = d.coord[sl.y], x3 = d.coord[sl.z];
-            V4 p1 = ld4(d.prim, sl.x), p2 = ld4(d.prim, sl.y), p3 = ld4(d.prim, sl.z);
-            double bx[3] = {x2.y - x3.y, x3.y - x1.y, x1.y - x2.y};
-            double by[3] = {x3.x - x2.x, x1.x - x3.x, x2.x - x1.x};
-            double dxt[4], dyt[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                dxt[i] = p1.v[i] * bx[0] + p2.v[i] * bx[1] + p3.v[i] * bx[2];
-                dyt[i] = p1.v[i] * by[0] + p2.v[i] * by[1] + p3.v[i] * by[2];
-            }
-            lx[k - cs] = make_double4(dxt[0], dxt[1], dxt[2], dxt[3]);
-            ly[k - cs] = make_double4(dyt[0], dyt[1], dyt[2], dyt[3]);
-            if (VISC) {
-                const int tri = sl.w >> 2, m = sl.w & 3;
-                const double n2b3 = 2.0 / 3.0;
-                double e1 = p1.v[3] / p1.v[0], e2 = p2.v[3] / p2.v[0], e3 = p3.v[3] / p3.v[0];
-                double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
-                double eyt = e1 * by[0] + e2 * by[1] + e3 * by[2];
-                double um = (p1.v[1] + p2.v[1] + p3.v[1]) / 3.0;
-                double vm = (p1.v[2] + p2.v[2] + p3.v[2]) / 3.0;
-                double mut = (d.mu[sl.x] + d.mu[sl.y] + d.mu[sl.z]) / 3.0;
-                double mult = (d.mul[sl.x] + d.mul[sl.y] + d.mul[sl.z]) / 3.0;
-                double prim5t = (d.nut[sl.x] + d.nut[sl.y] + d.nut[sl.z]) / 3.0;
-                double rhot = (p1.v[0] + p2.v[0] + p3.v[0]) / 3.0;
-                double chi = prim5t * rhot / mult;
-                double chi3 = pow3(chi);
-                double fv1 = chi3 / (chi3 + c.Cv11);
-                double txx = n2b3 * mut * (2.0 * dxt[1] - dyt[2]);
-                double txy = mut * (dyt[1] + dxt[2]);
-                double tyy = n2b3 * mut * (2.0 * dyt[2] - dxt[1]);
-                double gpr = c.GAMMA * (mult / c.prandtl + rhot * prim5t * fv1 / c.prandtl_turb) / c.GAMMA1;
-                double efx = txx * um + txy * vm + gpr * ext;
-                double efy = txy * um + tyy * vm + gpr * eyt;
-                double areat = 0.25 / d.elarea[tri];
-                double bxm = bx[m], bym = by[m];
-                lv[3 * (k - cs)] = (txx * bxm + txy * bym) * areat;
-                lv[3 * (k - cs) + 1] = (txy * bxm + tyy * bym) * areat;
-                lv[3 * (k - cs) + 2] = (efx * bxm + efy * bym) * areat;
-            }
-        }
-        __syncthreads();
-        if (own) {
-            const int kb = max(ks, cs), kz = min(ke, ce);
-            for (int k = kb; k < kz; ++k) {
-                double4 a = lx[k - cs], q = ly[k - cs];
-                gx[0] = gx[0] + a.x; gx[1] = gx[1] + a.y; gx[2] = gx[2] + a.z; gx[3] = gx[3] + a.w;
-                gy[0] = gy[0] + q.x; gy[1] = gy[1] + q.y; gy[2] = gy[2] + q.z; gy[3] = gy[3] + q.w;
-                if (VISC) {
-                    rv1 = rv1 + lv[3 * (k - cs)];
-                    rv2 = rv2 + lv[3 * (k - cs) + 1];
-                    rv3 = rv3 + lv[3 * (k - cs) + 2];
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (!own) return;
-    double ais = d.ais[v];
-    d.qx[v] = make_double4(gx[0] * ais, gx[1] * ais, gx[2] * ais, gx[3] * ais);
-    d.qy[v] = make_double4(gy[0] * ais, gy[1] * ais, gy[2] * ais, gy[3] * ais);
-    if (VISC) d.resv[v] = make_double4(0.0, rv1, rv2, rv3);
-}
-
-// Flux of one slot: an interior edge seen from one end (sign folded in) or a
-// boundary edge's contribution to one end.  false: Roe average failed.
-template <int FLUX, int LIM>
-__device__ __forceinline__ bool slot_flux(const Dev &d, const Consts &c, double cl, int s, V4 &f,
-                                          int &bad_edge)
-{
-    if (s >= kBndSlot) {
-        const int q = s - kBndSlot, bnd = q >> 1;
-        int2 e = d.bnode[bnd];
-        double2 x1 = d.coord[e.x], x2 = d.coord[e.y];
-        V4 r1, r2;
-        if (bnd < d.nsw) {
-            solid_flux(x1, x2, d.prim[e.x].w, d.prim[e.y].w, r1, r2);
-        } else {
-            V4 p1 = ld4(d.prim, e.x), p2 = ld4(d.prim, e.y);
-            farfield_flux(c, d.uout[bnd - d.nsw], x1, x2, p1, p2, r1, r2);
-        }
-        f = (q & 1) ? r2 : r1;
-        return true;
-    }
-    const int ed = (s > 0 ? s : -s) - 1;
-    int2 e = d.enode[ed];
-    double2 ds = d.eds[ed];
-    double2 x1 = d.coord[e.x], x2 = d.coord[e.y];
-    V4 p1 = ld4(d.prim, e.x), p2 = ld4(d.prim, e.y);
-    V4 qx1 = ld4(d.qx, e.x), qy1 = ld4(d.qy, e.x), qx2 = ld4(d.qx, e.y), qy2 = ld4(d.qy, e.y);
-    V4 pl, pr;
-    recon<LIM>(c, x2.x - x1.x, x2.y - x1.y, p1, p2, qx1, qy1, qx2, qy2, pl, pr);
-    bool ok = true;
-    if (FLUX == 1) ok = roe_flux(c, ds.x, ds.y, pl, pr, f);
-    else if (FLUX == 2) kfvs_flux(c, ds.x, ds.y, pl, pr, f);
-    else if (FLUX == 3) hllc_flux<0>(c, ds.x, ds.y, pl, pr, f);
-    else hllc_flux<1>(c, ds.x, ds.y, pl, pr, f);
-    if (!ok) bad_edge = ed + 1;
-    if (s < 0) {                        // r - f == r + (-f) exactly
-        f.v[0] = -f.v[0]; f.v[1] = -f.v[1]; f.v[2] = -f.v[2]; f.v[3] = -f.v[3];
-    }
-    return ok;
-}
-
-// Flux + RK stage.  MODE 0: stage IRK; MODE 1: last stage, which also
-// writes the conres partials, checks positivity and, unless SA follows,
-// starts the next iteration (save_old + time_step); MODE 2: residual only.
-template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
-__global__ void __launch_bounds__(kBlock) k_fu2(Dev d, Consts c, DevState *S, int irk, double4 *pout,
-                                                double4 *res_out)
-{
-    if (S->done || S->err_code) return;
-    const int b = logical_block(d.nblocks, d.xcd);
-    const int v0 = b * d.vb, v1 = min(v0 + d.vb, d.np);
-    const int t = threadIdx.x, v = v0 + t;
-    const bool own = v < v1;
-    const int s0 = d.v2s_ptr[v0], s1 = d.v2s_ptr[v1];
-    const int ks = own ? d.v2s_ptr[v] : 0, ke = own ? d.v2s_ptr[v + 1] : 0;
-    const double cl = S->cl;
-    (void)cl;
-    __shared__ double4 lf[kChunk];
-    V4 r;
-    if (VISC && own) r = ld4(d.resv, v);
-    else r.v[0] = r.v[1] = r.v[2] = r.v[3] = 0.0;
-    int bad = INT_MAX;
-    for (int cs = s0; cs < s1; cs += kChunk) {
-        const int ce = min(cs + kChunk, s1);
-        for (int k = cs + t; k < ce; k += kBlock) {
-            V4 f;
-            int be = 0;
-            if (!slot_flux<FLUX, LIM>(d, c, cl, d.sslot[k], f, be)) bad = min(bad, be);
-            st4(lf, k - cs, f);
-        }
-        __syncthreads();
-        if (own) {
-            const int kb = max(ks, cs), kz = min(ke, ce);
-            for (int k = kb; k < kz; ++k) {
-                double4 f = lf[k - cs];
-                r.v[0] = r.v[0] + f.x; r.v[1] = r.v[1] + f.y; r.v[2] = r.v[2] + f.z; r.v[3] = r.v[3] + f.w;
-            }
-        }
-        __syncthreads();
-    }
-    if (bad != INT_MAX) {
-        atomicMin(&S->err_index, bad);
-        S->err_code = FVM_E_NEG_SOUND;
-    }
-    if (MODE == 2) {
-        if (own) st4(res_out, v, r);
-        return;
-    }
-    Red red{0.0, 0.0, 0};
-    if (own) {
-        V4 po = ld4(d.prim_old, v), p = ld4(d.prim, v), u0, u1, un;
-        prim2con(c, po, u0);
-        prim2con(c, p, u1);
-        const double a = c.airk[irk - 1], bb = c.birk[irk - 1];
-        double f = d.dt[v] / d.cvarea[v];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) un.v[j] = a * u0.v[j] + bb * (u1.v[j] - f * r.v[j]);
-        if (VISC && d.solid[v]) { un.v[1] = 0.0; un.v[2] = 0.0; }
-        double4 q;
-        q.x = un.v[0];
-        q.y = un.v[1] / un.v[0];
-        q.z = un.v[2] / un.v[0];
-        q.w = c.GAMMA1 * (un.v[3] - 0.5 * (sq(un.v[1]) + sq(un.v[2])) / un.v[0]);
-        pout[v] = q;
-        if (MODE == 1) {
-            double gra = q.x - po.v[0];
-            double dif = fabs(gra);
-            red.s = sq(gra);
-            if (dif > 0.0) { red.m = dif; red.i = d.orig[v]; }
-            if (!TURB) {
-                if (q.x <= 0.0 || q.w <= 0.0) {
-                    atomicMin(&S->err_index, d.orig[v]);
-                    S->err_code = FVM_E_NONPOSITIVE;
-                }
-                // time_step of the next iteration (src/time_step.f:13-28)
-                double qq = sqrt(sq(q.y) + sq(q.z));
-                double aa = sqrt(c.GAMMA * q.w / q.x);
-                double dr = d.drmin[v];
-                if (VISC) {
-                    double dtv = 2.0 * c.GAMMA * d.mu[v] / (q.x * c.prandtl);
-                    d.dt[v] = c.cfl * sq(dr) / (dr * (qq + aa) + dtv);
-                } else {
-                    d.dt[v] = c.cfl * dr / (qq + aa);
-                }
-            }
-        }
-    }
-    if (MODE == 1) {
-        Red rr = block_reduce(red);
-        if (t == 0) {
-            d.partial[3 * b] = rr.s;
-            d.partial[3 * b + 1] = rr.m;
-            d.partial[3 * b + 2] = (double)rr.i;
-        }
-    }
-}
-
 // ------------------------------------------------- patch stage kernel
-template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
 __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, PatchArgs pa,
                                                   int irk, const double4 *__restrict__ pin,
                                                   double4 *__restrict__ pout, double4 *res_out)
@@ -878,13 +649,30 @@ __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, 
         V4 p1{{a1.x, a1.y, a1.z, a1.w}}, p2{{a2.x, a2.y, a2.z, a2.w}};
         V4 qx1{{g1.x, g1.y, g1.z, g1.w}}, qy1{{h1.x, h1.y, h1.z, h1.w}};
         V4 qx2{{g2.x, g2.y, g2.z, g2.w}}, qy2{{h2.x, h2.y, h2.z, h2.w}};
-        V4 pl, pr, f;
-        recon<LIM>(c, x2.x - x1.x, x2.y - x1.y, p1, p2, qx1, qy1, qx2, qy2, pl, pr);
+        V4 f;
         bool ok = true;
-        if (FLUX == 1) ok = roe_flux(c, ds.x, ds.y, pl, pr, f);
-        else if (FLUX == 2) kfvs_flux(c, ds.x, ds.y, pl, pr, f);
-        else if (FLUX == 3) hllc_flux<0>(c, ds.x, ds.y, pl, pr, f);
-        else hllc_flux<1>(c, ds.x, ds.y, pl, pr, f);
+        if (PREC == 0) {
+            V4 pl, pr;
+            recon<LIM>(c, x2.x - x1.x, x2.y - x1.y, p1, p2, qx1, qy1, qx2, qy2, pl, pr);
+            if (FLUX == 1) ok = roe_flux(c, ds.x, ds.y, pl, pr, f);
+            else if (FLUX == 2) kfvs_flux(c, ds.x, ds.y, pl, pr, f);
+            else if (FLUX == 3) hllc_flux<0>(c, ds.x, ds.y, pl, pr, f);
+            else hllc_flux<1>(c, ds.x, ds.y, pl, pr, f);
+        } else {
+            // split precision: reconstruction and flux in FP32 on FP64 inputs
+            // (edge vector from FP64 coordinates); the flux returns to FP64
+            // for the accumulation
+            V4f pl, pr, ff;
+            const float dxf = (float)(x2.x - x1.x), dyf = (float)(x2.y - x1.y);
+            recon<LIM, float>(c, dxf, dyf, to_f(p1), to_f(p2), to_f(qx1), to_f(qy1), to_f(qx2),
+                              to_f(qy2), pl, pr);
+            const float sxf = (float)ds.x, syf = (float)ds.y;
+            if (FLUX == 1) ok = roe_flux<float>(c, sxf, syf, pl, pr, ff);
+            else if (FLUX == 2) kfvs_flux<float>(c, sxf, syf, pl, pr, ff);
+            else if (FLUX == 3) hllc_flux<0, float>(c, sxf, syf, pl, pr, ff);
+            else hllc_flux<1, float>(c, sxf, syf, pl, pr, ff);
+            f = to_d(ff);
+        }
         if (!ok) bad = min(bad, ge + 1);
         reinterpret_cast<double4 *>(s_work)[k] = make_double4(f.v[0], f.v[1], f.v[2], f.v[3]);
     }
@@ -981,6 +769,46 @@ __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, 
     }
 }
 
+// Wall distance, src/wall_dist.f:28-61: for every point the smallest
+// distance to the end points of the solid edges and, where the foot of the
+// perpendicular falls inside an edge, to the edge.  The solid edges are
+// staged through LDS in tiles; fmin over the same candidates gives the same
+// value in any order.  COMPAT keeps the reference's dy3 = x2 - x1
+// (src/wall_dist.f:46).
+template <bool COMPAT>
+__global__ void __launch_bounds__(kBlock) k_wall_dist(int np, const double2 *__restrict__ xy,
+                                                     const int2 *__restrict__ sedge, int nse,
+                                                     const unsigned char *__restrict__ solid,
+                                                     double *__restrict__ wd)
+{
+    __shared__ double2 ea[kBlock], eb[kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const double2 x = i < np ? xy[i] : make_double2(0.0, 0.0);
+    double w = 1.0e10;
+    for (int base = 0; base < nse; base += kBlock) {
+        const int j = base + threadIdx.x;
+        if (j < nse) { int2 e = sedge[j]; ea[threadIdx.x] = xy[e.x]; eb[threadIdx.x] = xy[e.y]; }
+        __syncthreads();
+        const int n = min(kBlock, nse - base);
+        for (int k = 0; k < n; ++k) {
+            const double2 a = ea[k], b = eb[k];
+            const double dx1 = a.x - x.x, dy1 = a.y - x.y;
+            w = fmin(w, sqrt(dx1 * dx1 + dy1 * dy1));
+            const double dx2 = b.x - x.x, dy2 = b.y - x.y;
+            w = fmin(w, sqrt(dx2 * dx2 + dy2 * dy2));
+            const double dx3 = b.x - a.x;
+            const double dy3 = COMPAT ? b.x - a.x : b.y - a.y;
+            const double xi = -(dx1 * dx3 + dy1 * dy3) / (dx3 * dx3 + dy3 * dy3);
+            if (xi >= 0.0 && xi <= 1.0) {
+                const double xp = a.x + xi * dx3, yp = a.y + xi * dy3;
+                w = fmin(w, sqrt((x.x - xp) * (x.x - xp) + (x.y - yp) * (x.y - yp)));
+            }
+        }
+        __syncthreads();
+    }
+    if (i < np) wd[i] = solid[i] ? 0.0 : w;
+}
+
 __global__ void k_begin_chunk(DevState *S) { S->nhist = 0; }
 
 // far-field states of all farfield edges for a given cl
@@ -1013,7 +841,7 @@ struct fvm_ctx {
     double4 *res_tmp = nullptr;
     double4 *prim_bak = nullptr;
     double *nut_bak = nullptr;
-    int variant = 3;                  // 1: edge-flux buffer; 2: slot-parallel; 3: patch stage kernel
+    int variant = 3;                  // 1: edge-flux buffer + gathers; 3: patch stage kernel
     double4 *pbuf[3] = {nullptr, nullptr, nullptr};   // stage buffers O, B, C
     PatchArgs pa{};
     std::vector<int> perm;            // device vertex -> caller's vertex (0-based)
@@ -1172,17 +1000,14 @@ Dev stage_dev(const fvm_ctx *x, const double4 *pin)
 }
 
 template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
-void launch_fu2_t(fvm_ctx *x, DevState *S, int irk, const double4 *pin, double4 *pout, double4 *res)
-{
-    hipLaunchKernelGGL((k_fu2<FLUX, LIM, VISC, MODE, TURB>), dim3(x->d.nblocks), dim3(kBlock), 0,
-                       x->stream, stage_dev(x, pin), x->c, S, irk, pout, res);
-}
-
-template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
 void launch_patch_t(fvm_ctx *x, DevState *S, int irk, const double4 *pin, double4 *pout, double4 *res)
 {
-    hipLaunchKernelGGL((k_patch<FLUX, LIM, VISC, MODE, TURB>), dim3(x->pa.npatch), dim3(kBlock),
-                       x->pa.lds_bytes, x->stream, x->d, x->c, S, x->pa, irk, pin, pout, res);
+    if (x->par.precision == 1)
+        hipLaunchKernelGGL((k_patch<FLUX, LIM, VISC, MODE, TURB, 1>), dim3(x->pa.npatch), dim3(kBlock),
+                           x->pa.lds_bytes, x->stream, x->d, x->c, S, x->pa, irk, pin, pout, res);
+    else
+        hipLaunchKernelGGL((k_patch<FLUX, LIM, VISC, MODE, TURB, 0>), dim3(x->pa.npatch), dim3(kBlock),
+                           x->pa.lds_bytes, x->stream, x->d, x->c, S, x->pa, irk, pin, pout, res);
 }
 
 template <template <int, int, bool, int, bool> class L, int FLUX, int LIM>
@@ -1223,39 +1048,15 @@ void dispatch(fvm_ctx *x, DevState *S, int irk, int mode, const double4 *pin, do
 }
 
 template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
-struct FuLaunch {
-    static void go(fvm_ctx *x, DevState *S, int irk, const double4 *pin, double4 *pout, double4 *res)
-    { launch_fu2_t<FLUX, LIM, VISC, MODE, TURB>(x, S, irk, pin, pout, res); }
-};
-template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
 struct PatchLaunch {
     static void go(fvm_ctx *x, DevState *S, int irk, const double4 *pin, double4 *pout, double4 *res)
     { launch_patch_t<FLUX, LIM, VISC, MODE, TURB>(x, S, irk, pin, pout, res); }
 };
 
-void launch_fu2(fvm_ctx *x, DevState *S, int irk, int mode, const double4 *pin, double4 *pout,
-                double4 *res)
-{
-    dispatch<FuLaunch>(x, S, irk, mode, pin, pout, res);
-}
-
 void launch_patch(fvm_ctx *x, DevState *S, int irk, int mode, const double4 *pin, double4 *pout,
                   double4 *res)
 {
     dispatch<PatchLaunch>(x, S, irk, mode, pin, pout, res);
-}
-
-void launch_visc(fvm_ctx *x, DevState *S, const double4 *pin)
-{
-    const int np = x->np;
-    if (x->par.iflow == 3) hipLaunchKernelGGL(k_visc<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, stage_dev(x, pin), x->c, S);
-    else hipLaunchKernelGGL(k_visc<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, stage_dev(x, pin), x->c, S);
-}
-
-void launch_grad2(fvm_ctx *x, DevState *S, const double4 *pin)
-{
-    if (x->par.iflow != 1) hipLaunchKernelGGL(k_grad2<true>, dim3(x->d.nblocks), dim3(kBlock), 0, x->stream, stage_dev(x, pin), x->c, S);
-    else hipLaunchKernelGGL(k_grad2<false>, dim3(x->d.nblocks), dim3(kBlock), 0, x->stream, stage_dev(x, pin), x->c, S);
 }
 
 void launch_tstep(fvm_ctx *x)
@@ -1289,7 +1090,7 @@ void one_iteration(fvm_ctx *x, Tally *tally)
         tally->ms[k] += ms;
         tally->cnt[k] += 1;
     };
-    if (x->variant >= 2) {
+    if (x->variant == 3) {
         // time_step runs at the end of the previous iteration's last stage,
         // except when the SA step (which reads this iteration's dt) follows
         if (turb) timed("k_tstep", [&] { launch_tstep(x); });
@@ -1297,13 +1098,7 @@ void one_iteration(fvm_ctx *x, Tally *tally)
             const double4 *pin = x->pbuf[irk - 1];
             double4 *pout = x->pbuf[irk % 3];
             const int mode = irk == 3 ? 1 : 0;
-            if (x->variant == 3) {
-                timed("k_patch", [&] { launch_patch(x, x->S, irk, mode, pin, pout, nullptr); });
-            } else {
-                if (visc) timed("k_visc", [&] { launch_visc(x, x->S, pin); });
-                timed("k_grad2", [&] { launch_grad2(x, x->S, pin); });
-                timed("k_fu2", [&] { launch_fu2(x, x->S, irk, mode, pin, pout, nullptr); });
-            }
+            timed("k_patch", [&] { launch_patch(x, x->S, irk, mode, pin, pout, nullptr); });
         }
     } else {
     timed("k_tstep", [&] { launch_tstep(x); });
@@ -1347,13 +1142,14 @@ void one_iteration(fvm_ctx *x, Tally *tally)
 // dt = time_step(prim)) that the variant-2 last stage maintains.
 int configure(fvm_ctx *x, int variant, int vb, int xcd)
 {
-    if (variant < 1 || variant > 3) return FVM_E_INPUT;
+    if (variant != 1 && variant != 3) return FVM_E_INPUT;
+    if (variant == 1 && x->par.precision != 0) return FVM_E_INPUT;   // split precision: stage kernel only
     if (vb < 16 || vb > kBlock) return FVM_E_INPUT;
     x->variant = variant;
     x->d.vb = vb;
     x->d.nblocks = (x->np + vb - 1) / vb;
     x->d.xcd = xcd ? 1 : 0;
-    x->d.nparts = variant == 3 ? x->pa.npatch : (variant == 2 ? x->d.nblocks : (int)nblk(x->np));
+    x->d.nparts = variant == 3 ? x->pa.npatch : (int)nblk(x->np);
     if (x->graph_n) (void)hipGraphExecDestroy(x->graph_n);
     if (x->graph_1) (void)hipGraphExecDestroy(x->graph_1);
     x->graph_n = x->graph_1 = nullptr;
@@ -1400,7 +1196,7 @@ int check_params(const fvm_params *p)
     if (p->ilimit != 0 && p->ilimit != 1) return FVM_E_INPUT;
     if (p->vortex != 0 && p->vortex != 1) return FVM_E_INPUT;
     if (p->cell_type != 1 && p->cell_type != 2) return FVM_E_INPUT;
-    if (p->precision != 0) return FVM_E_INPUT;
+    if (p->precision != 0 && p->precision != 1) return FVM_E_INPUT;
     return FVM_OK;
 }
 
@@ -1480,9 +1276,9 @@ struct PatchHost {
     int lds_max = 0;
 };
 
-// dynamic LDS per workgroup; the static LDS of the block reduction (5 KiB)
-// comes on top, within the 64 KiB a workgroup may address
-constexpr int kLdsLimit = 56 * 1024;
+// dynamic LDS per workgroup when the device is not asked: 64 KiB minus the
+// static LDS of the block reduction (5 KiB) and a margin
+constexpr int kLdsDefault = 58 * 1024;
 
 int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<int> &aptr,
                   const std::vector<int> &adj, const std::vector<int> &v2t_ptr,
@@ -1490,7 +1286,7 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
                   const std::vector<int> &v2e_ptr, const std::vector<int> &v2e,
                   const std::vector<int2> &enode, const std::vector<int> &v2b_ptr,
                   const std::vector<int> &v2b, const std::vector<int2> &bnode, bool visc,
-                  PatchHost &ph)
+                  int lds_limit, PatchHost &ph)
 {
     const int ntri = (int)tri.size(), nedge = (int)enode.size(), nbnd = (int)bnode.size();
     std::vector<int> loc(np, -1), tloc(ntri, -1), eloc(nedge, -1), bloc(nbnd, -1);
@@ -1595,7 +1391,7 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
         for (int e : el) eloc[e] = -1;
         for (int b : bl) bloc[b] = -1;
     }
-    return ph.lds_max > kLdsLimit ? FVM_E_SIZE : FVM_OK;
+    return ph.lds_max > lds_limit ? FVM_E_SIZE : FVM_OK;
 }
 
 
@@ -1606,15 +1402,15 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
 struct Layout {
     std::vector<int> perm, iperm, patch_ptr, nadj_ptr, nadj;
     std::vector<double2> coord, eds;
-    std::vector<int4> tri, tslot;
-    std::vector<int> v2t_ptr, v2t, v2e_ptr, v2e, v2b_ptr, v2b, v2s_ptr, sslot, orig;
+    std::vector<int4> tri;
+    std::vector<int> v2t_ptr, v2t, v2e_ptr, v2e, v2b_ptr, v2b, orig;
     std::vector<int2> enode, bnode;
     std::vector<double> ais, cvarea, drmin;
     std::vector<unsigned char> solid;
     PatchHost ph;
 };
 
-int build_layout(const fvm_mesh *m, int iflow, int patch_size, Layout &L)
+int build_layout(const fvm_mesh *m, int iflow, int patch_size, int lds_limit, Layout &L)
 {
     const int np = m->np, nt = m->nt, ne = m->ne;
     const int nsw = std::max(0, m->nswe2 - m->nswe1 + 1), nff = std::max(0, m->nffe2 - m->nffe1 + 1);
@@ -1691,23 +1487,6 @@ int build_layout(const fvm_mesh *m, int iflow, int patch_size, Layout &L)
         std::vector<int> f(v2b_ptr.begin(), v2b_ptr.end() - 1);
         for (int b = 0; b < nb; ++b) { v2b[f[bnode[b].x]++] = 2 * b; v2b[f[bnode[b].y]++] = 2 * b + 1; }
     }
-    // slot tables: (vertex, triangle) slots with the triangle's nodes and the
-    // vertex's position in it; flux slots = edge slots then boundary slots
-    std::vector<int4> tslot(3 * (size_t)nt);
-    for (int v = 0; v < np; ++v)
-        for (int k = v2t_ptr[v]; k < v2t_ptr[v + 1]; ++k) {
-            int t = v2t[k];
-            int4 n = tri[t];
-            int mm = (n.x == v) ? 0 : ((n.y == v) ? 1 : 2);
-            tslot[k] = make_int4(n.x, n.y, n.z, 4 * t + mm);
-        }
-    std::vector<int> v2s_ptr(np + 1, 0), sslot;
-    sslot.reserve(v2e.size() + v2b.size());
-    for (int v = 0; v < np; ++v) {
-        for (int k = v2e_ptr[v]; k < v2e_ptr[v + 1]; ++k) sslot.push_back(v2e[k]);
-        for (int k = v2b_ptr[v]; k < v2b_ptr[v + 1]; ++k) sslot.push_back(kBndSlot + v2b[k]);
-        v2s_ptr[v + 1] = (int)sslot.size();
-    }
     std::vector<double> ais(np), cvarea(np), drmin(np);
     const double n1b6 = 1.0 / 6.0;
     for (int n = 0; n < np; ++n) {
@@ -1725,20 +1504,20 @@ int build_layout(const fvm_mesh *m, int iflow, int patch_size, Layout &L)
     for (;;) {
         L.ph = PatchHost();
         int rc = build_patches(np, part.patch_ptr, nadj_ptr, nadj, v2t_ptr, v2t, tri, v2e_ptr, v2e,
-                               enode, v2b_ptr, v2b, bnode, iflow != 1, L.ph);
+                               enode, v2b_ptr, v2b, bnode, iflow != 1, lds_limit, L.ph);
         if (rc == FVM_OK) break;
         if (rc != FVM_E_SIZE || target <= 16) return rc;
         target /= 2;
         part = fvm::partition_vertices(np, adj_ptr, adj, target);
         // renumbering changed: rebuild everything for the smaller patches
-        return build_layout(m, iflow, target, L);
+        return build_layout(m, iflow, target, lds_limit, L);
     }
     L.perm = std::move(perm); L.iperm = std::move(iperm); L.patch_ptr = std::move(part.patch_ptr);
     L.nadj_ptr = std::move(nadj_ptr); L.nadj = std::move(nadj);
-    L.coord = std::move(coord); L.eds = std::move(eds); L.tri = std::move(tri); L.tslot = std::move(tslot);
+    L.coord = std::move(coord); L.eds = std::move(eds); L.tri = std::move(tri);
     L.v2t_ptr = std::move(v2t_ptr); L.v2t = std::move(v2t); L.v2e_ptr = std::move(v2e_ptr);
     L.v2e = std::move(v2e); L.v2b_ptr = std::move(v2b_ptr); L.v2b = std::move(v2b);
-    L.v2s_ptr = std::move(v2s_ptr); L.sslot = std::move(sslot); L.orig = std::move(orig);
+    L.orig = std::move(orig);
     L.enode = std::move(enode); L.bnode = std::move(bnode); L.ais = std::move(ais);
     L.cvarea = std::move(cvarea); L.drmin = std::move(drmin); L.solid = std::move(solid);
     return FVM_OK;
@@ -1877,12 +1656,12 @@ long long fvm_gpu_footprint_estimate(int np, int nt, int netot, int iflow, int p
     return b;
 }
 
-int fvm_patch_plan(const fvm_mesh *m, int iflow, int patch_size, long long *st)
+int fvm_patch_plan(const fvm_mesh *m, int iflow, int patch_size, int lds_limit, long long *st)
 {
     for (int i = 0; i < 8; ++i) st[i] = 0;
     if (m->np <= 0 || m->nt <= 0 || m->ne <= 0) return FVM_E_INPUT;
     Layout L;
-    int rc = build_layout(m, iflow, patch_size, L);
+    int rc = build_layout(m, iflow, patch_size, lds_limit > 0 ? lds_limit : kLdsDefault, L);
     if (rc) return rc;
     const PatchHost &ph = L.ph;
     const int npatch = (int)L.patch_ptr.size() - 1;
@@ -1920,8 +1699,12 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     set_consts(x);
     if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) { delete x; return FVM_E_HIP; }
 
+    // LDS a workgroup may use on this device, less the static LDS
+    int dev = 0, lds_dev = 64 * 1024;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     Layout L;
-    rc = build_layout(m, par->iflow, par->patch_size, L);
+    rc = build_layout(m, par->iflow, par->patch_size, lds_dev - 6 * 1024, L);
     if (!rc && validate_layout(m, L)) rc = FVM_E_INPUT;
     if (rc) { fvm_gpu_destroy(x); return rc; }
     x->perm = L.perm;
@@ -1944,9 +1727,6 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     TRY(upload(x, (int2 **)&d.bnode, L.bnode));
     TRY(upload(x, (int **)&d.v2b_ptr, L.v2b_ptr));
     TRY(upload(x, (int **)&d.v2b, L.v2b));
-    TRY(upload(x, (int4 **)&d.tslot, L.tslot));
-    TRY(upload(x, (int **)&d.v2s_ptr, L.v2s_ptr));
-    TRY(upload(x, (int **)&d.sslot, L.sslot));
     TRY(upload(x, (double **)&d.ais, L.ais));
     TRY(upload(x, (double **)&d.cvarea, L.cvarea));
     TRY(upload(x, (double **)&d.drmin, L.drmin));
@@ -2034,6 +1814,69 @@ int fvm_gpu_destroy(fvm_ctx *x)
     return FVM_OK;
 }
 
+
+int fvm_gpu_wall_dist(int np, const double *coord, const int *ptype, int netot, const int *edge,
+                      int sa_compat, double *wd, double *wd1)
+{
+    if (np <= 0 || netot <= 0) return FVM_E_INPUT;
+    std::vector<int2> se;
+    for (int e = 0; e < netot; ++e) {
+        int a = edge[2 * e] - 1, b = edge[2 * e + 1] - 1;
+        if (a < 0 || a >= np || b < 0 || b >= np) return FVM_E_INPUT;
+        if (ptype[a] == 3 && ptype[b] == 3) se.push_back(make_int2(a, b));
+    }
+    std::vector<unsigned char> solid(np);
+    for (int p = 0; p < np; ++p) solid[p] = ptype[p] == 3;
+    double2 *dxy = nullptr; int2 *dse = nullptr; unsigned char *dsol = nullptr; double *dwd = nullptr;
+    hipStream_t st = nullptr;
+    int rc = FVM_OK;
+    auto cleanup = [&] {
+        if (st) (void)hipStreamSynchronize(st);
+        (void)hipFree(dxy); (void)hipFree(dse); (void)hipFree(dsol); (void)hipFree(dwd);
+        if (st) (void)hipStreamDestroy(st);
+    };
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void **)&dxy, sizeof(double2) * np) != hipSuccess ||
+        hipMalloc((void **)&dse, sizeof(int2) * std::max<size_t>(se.size(), 1)) != hipSuccess ||
+        hipMalloc((void **)&dsol, np) != hipSuccess ||
+        hipMalloc((void **)&dwd, sizeof(double) * np) != hipSuccess) {
+        cleanup();
+        return FVM_E_NOMEM;
+    }
+    if (hipMemcpyAsync(dxy, coord, sizeof(double2) * np, hipMemcpyHostToDevice, st) != hipSuccess ||
+        (!se.empty() && hipMemcpyAsync(dse, se.data(), sizeof(int2) * se.size(), hipMemcpyHostToDevice, st) != hipSuccess) ||
+        hipMemcpyAsync(dsol, solid.data(), np, hipMemcpyHostToDevice, st) != hipSuccess) {
+        cleanup();
+        return FVM_E_HIP;
+    }
+    if (sa_compat)
+        hipLaunchKernelGGL(k_wall_dist<true>, dim3(nblk(np)), dim3(kBlock), 0, st, np, dxy, dse, (int)se.size(), dsol, dwd);
+    else
+        hipLaunchKernelGGL(k_wall_dist<false>, dim3(nblk(np)), dim3(kBlock), 0, st, np, dxy, dse, (int)se.size(), dsol, dwd);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(wd, dwd, sizeof(double) * np, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        rc = FVM_E_HIP;
+    cleanup();
+    if (rc) return rc;
+    // second layer (src/wall_dist.f:64-73): nearest non-solid neighbour of
+    // every solid point, in the solid points' order; then solid points -> 1
+    if (wd1) {
+        std::vector<double> w1(np, 1.0e10);
+        for (int e = 0; e < netot; ++e) {
+            int a = edge[2 * e] - 1, b = edge[2 * e + 1] - 1;
+            if (solid[a] && !solid[b]) w1[a] = std::fmin(w1[a], wd[b]);
+            if (solid[b] && !solid[a]) w1[b] = std::fmin(w1[b], wd[a]);
+        }
+        int k = 0;
+        for (int p = 0; p < np; ++p)
+            if (solid[p]) wd1[k++] = w1[p];
+    }
+    for (int p = 0; p < np; ++p)
+        if (solid[p]) wd[p] = 1.0;
+    return FVM_OK;
+}
+
 int fvm_gpu_tune(fvm_ctx *x, int variant, int vertices_per_block, int xcd_remap)
 {
     return configure(x, variant, vertices_per_block, xcd_remap);
@@ -2089,10 +1932,6 @@ int fvm_gpu_residu(fvm_ctx *x, const double *prim, const double *nut, double *re
     HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
     if (x->variant == 3) {
         launch_patch(x, x->S_hook, 1, 2, x->d.prim, x->pbuf[1], x->res_tmp);
-    } else if (x->variant == 2) {
-        if (x->par.iflow != 1) launch_visc(x, x->S_hook, x->d.prim);
-        launch_grad2(x, x->S_hook, x->d.prim);
-        launch_fu2(x, x->S_hook, 1, 2, x->d.prim, x->pbuf[1], x->res_tmp);
     } else {
         residual_front(x, x->S_hook);
         if (x->par.iflow != 1)

@@ -27,7 +27,9 @@ __device__ __forceinline__ double sq(double x) { return x * x; }
 __device__ __forceinline__ double pow3(double x) { double x2 = x * x; return x * x2; }
 __device__ __forceinline__ double pow6(double x) { double x2 = x * x; double x4 = x2 * x2; return x2 * x4; }
 
-struct V4 { double v[4]; };
+template <class T> struct VT { T v[4]; };
+using V4 = VT<double>;
+using V4f = VT<float>;
 __device__ __forceinline__ V4 ld4(const double4 *p, int i)
 {
     double4 a = p[i];
@@ -37,76 +39,91 @@ __device__ __forceinline__ void st4(double4 *p, int i, const V4 &a)
 {
     p[i] = make_double4(a.v[0], a.v[1], a.v[2], a.v[3]);
 }
+__device__ __forceinline__ V4f to_f(const V4 &a)
+{
+    return V4f{{(float)a.v[0], (float)a.v[1], (float)a.v[2], (float)a.v[3]}};
+}
+__device__ __forceinline__ V4 to_d(const V4f &a)
+{
+    return V4{{(double)a.v[0], (double)a.v[1], (double)a.v[2], (double)a.v[3]}};
+}
+
+// The reconstruction and the Riemann fluxes are templates on the arithmetic
+// type: T = double is the reference's arithmetic, T = float the split-
+// precision mode (geometry, accumulation, update and norms stay FP64).
 
 // van Albada limiter, src/recon.f:55-71
-__device__ __forceinline__ double limit_albada(double ul, double ur)
+template <class T>
+__device__ __forceinline__ T limit_albada(T ul, T ur)
 {
-    if (ul * ur <= 0.0) return 0.0;
-    double top = (sq(ul) + kEps) * ur + (sq(ur) + kEps) * ul;
-    double bot = sq(ul) + sq(ur) + 2.0 * kEps;
+    if (ul * ur <= T(0)) return T(0);
+    T top = (ul * ul + T(kEps)) * ur + (ur * ur + T(kEps)) * ul;
+    T bot = ul * ul + ur * ur + T(2.0 * kEps);
     return top / bot;
 }
 
 // MUSCL states at the edge midpoint, src/recon.f:15-44
-template <int LIM>
-__device__ __forceinline__ void recon(const Consts &c, double dx, double dy, const V4 &p1,
-                                      const V4 &p2, const V4 &qx1, const V4 &qy1,
-                                      const V4 &qx2, const V4 &qy2, V4 &pl, V4 &pr)
+template <int LIM, class T = double>
+__device__ __forceinline__ void recon(const Consts &c, T dx, T dy, const VT<T> &p1,
+                                      const VT<T> &p2, const VT<T> &qx1, const VT<T> &qy1,
+                                      const VT<T> &qx2, const VT<T> &qy2, VT<T> &pl, VT<T> &pr)
 {
+    const T A11 = T(c.ALBADA11), A12 = T(c.ALBADA12), A21 = T(c.ALBADA21), A22 = T(c.ALBADA22);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        double dqp = p2.v[i] - p1.v[i];
+        T dqp = p2.v[i] - p1.v[i];
         if (LIM) {
-            double dql = c.ALBADA21 * (dx * qx1.v[i] + dy * qy1.v[i]) + c.ALBADA22 * dqp;
-            pl.v[i] = p1.v[i] + 0.5 * limit_albada(dql, dqp);
-            double dqr = c.ALBADA21 * (dx * qx2.v[i] + dy * qy2.v[i]) + c.ALBADA22 * dqp;
-            pr.v[i] = p2.v[i] - 0.5 * limit_albada(dqr, dqp);
+            T dql = A21 * (dx * qx1.v[i] + dy * qy1.v[i]) + A22 * dqp;
+            pl.v[i] = p1.v[i] + T(0.5) * limit_albada<T>(dql, dqp);
+            T dqr = A21 * (dx * qx2.v[i] + dy * qy2.v[i]) + A22 * dqp;
+            pr.v[i] = p2.v[i] - T(0.5) * limit_albada<T>(dqr, dqp);
         } else {
-            double dql = dx * qx1.v[i] + dy * qy1.v[i];
-            pl.v[i] = p1.v[i] + 0.5 * (c.ALBADA11 * dql + c.ALBADA12 * dqp);
-            double dqr = dx * qx2.v[i] + dy * qy2.v[i];
-            pr.v[i] = p2.v[i] - 0.5 * (c.ALBADA11 * dqr + c.ALBADA12 * dqp);
+            T dql = dx * qx1.v[i] + dy * qy1.v[i];
+            pl.v[i] = p1.v[i] + T(0.5) * (A11 * dql + A12 * dqp);
+            T dqr = dx * qx2.v[i] + dy * qy2.v[i];
+            pr.v[i] = p2.v[i] - T(0.5) * (A11 * dqr + A12 * dqp);
         }
     }
 }
 
 // Roe flux without entropy fix, src/roe_flux.f:18-112.  false: a^2 <= 0.
-__device__ __forceinline__ bool roe_flux(const Consts &c, double dsx, double dsy, const V4 &L,
-                                         const V4 &R, V4 &f)
+template <class T = double>
+__device__ __forceinline__ bool roe_flux(const Consts &c, T dsx, T dsy, const VT<T> &L,
+                                         const VT<T> &R, VT<T> &f)
 {
-    const double G = c.GAMMA, G1 = c.GAMMA1;
-    double length = sqrt(sq(dsx) + sq(dsy));
-    double ct = dsx / length, st = dsy / length;
-    double rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
-    double al2 = G * pl / rl, ql2 = sq(ul) + sq(vl), hl = al2 / G1 + 0.5 * ql2;
-    double rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
-    double ar2 = G * pr / rr, qr2 = sq(ur) + sq(vr), hr = ar2 / G1 + 0.5 * qr2;
-    double unl = ul * ct + vl * st;
-    double rl12 = sqrt(rl), rr12 = sqrt(rr), rd = 1.0 / (rl12 + rr12);
-    double ua = (ul * rl12 + ur * rr12) * rd;
-    double va = (vl * rl12 + vr * rr12) * rd;
-    double ha = (hl * rl12 + hr * rr12) * rd;
-    double qa2 = sq(ua) + sq(va);
-    double aa2 = G1 * (ha - 0.5 * qa2);
-    if (!(aa2 > 0.0)) return false;
-    double aa = sqrt(aa2);
-    double una = ua * ct + va * st;
-    double vna = -ua * st + va * ct;
-    double l1 = fmin(una - aa, 0.0), l2 = fmin(una, 0.0), l4 = fmin(una + aa, 0.0);
-    double du1 = rr - rl, du2 = rr * ur - rl * ul, du3 = rr * vr - rl * vl;
-    double du4 = (pr - pl) / G1 + 0.5 * (rr * qr2 - rl * ql2);
-    double m1 = (ct * du2 + st * du3 - una * du1) / aa;
-    double m2 = G1 * (du4 - ua * du2 - va * du3 + qa2 * du1) / sq(aa);
-    double a4 = 0.5 * (m1 + m2), a1 = 0.5 * (m2 - m1), a3 = du1 - a1 - a4;
-    double a2 = (st * du2 - ct * du3 + vna * du1) / aa;
-    double a1l1 = a1 * l1, a2l2 = a2 * l2, a3l3 = a3 * l2, a4l4 = a4 * l4;
-    double aact = aa * ct, aast = aa * st;
-    double Fc0 = rl * unl, Fc1 = pl * ct + rl * ul * unl, Fc2 = pl * st + rl * vl * unl,
-           Fc3 = rl * hl * unl;
-    double Fd0 = a1l1 + a3l3 + a4l4;
-    double Fd1 = a1l1 * (ua - aact) + a2l2 * aa * st + a3l3 * ua + a4l4 * (ua + aact);
-    double Fd2 = a1l1 * (va - aast) - a2l2 * aa * ct + a3l3 * va + a4l4 * (va + aast);
-    double Fd3 = a1l1 * (ha - una * aa) + a2l2 * aa * vna + a3l3 * 0.5 * qa2 + a4l4 * (ha + una * aa);
+    const T G = T(c.GAMMA), G1 = T(c.GAMMA1), h = T(0.5);
+    T length = sqrt(dsx * dsx + dsy * dsy);
+    T ct = dsx / length, st = dsy / length;
+    T rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
+    T al2 = G * pl / rl, ql2 = ul * ul + vl * vl, hl = al2 / G1 + h * ql2;
+    T rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
+    T ar2 = G * pr / rr, qr2 = ur * ur + vr * vr, hr = ar2 / G1 + h * qr2;
+    T unl = ul * ct + vl * st;
+    T rl12 = sqrt(rl), rr12 = sqrt(rr), rd = T(1) / (rl12 + rr12);
+    T ua = (ul * rl12 + ur * rr12) * rd;
+    T va = (vl * rl12 + vr * rr12) * rd;
+    T ha = (hl * rl12 + hr * rr12) * rd;
+    T qa2 = ua * ua + va * va;
+    T aa2 = G1 * (ha - h * qa2);
+    if (!(aa2 > T(0))) return false;
+    T aa = sqrt(aa2);
+    T una = ua * ct + va * st;
+    T vna = -ua * st + va * ct;
+    T l1 = fmin(una - aa, T(0)), l2 = fmin(una, T(0)), l4 = fmin(una + aa, T(0));
+    T du1 = rr - rl, du2 = rr * ur - rl * ul, du3 = rr * vr - rl * vl;
+    T du4 = (pr - pl) / G1 + h * (rr * qr2 - rl * ql2);
+    T m1 = (ct * du2 + st * du3 - una * du1) / aa;
+    T m2 = G1 * (du4 - ua * du2 - va * du3 + qa2 * du1) / (aa * aa);
+    T a4 = h * (m1 + m2), a1 = h * (m2 - m1), a3 = du1 - a1 - a4;
+    T a2 = (st * du2 - ct * du3 + vna * du1) / aa;
+    T a1l1 = a1 * l1, a2l2 = a2 * l2, a3l3 = a3 * l2, a4l4 = a4 * l4;
+    T aact = aa * ct, aast = aa * st;
+    T Fc0 = rl * unl, Fc1 = pl * ct + rl * ul * unl, Fc2 = pl * st + rl * vl * unl,
+      Fc3 = rl * hl * unl;
+    T Fd0 = a1l1 + a3l3 + a4l4;
+    T Fd1 = a1l1 * (ua - aact) + a2l2 * aa * st + a3l3 * ua + a4l4 * (ua + aact);
+    T Fd2 = a1l1 * (va - aast) - a2l2 * aa * ct + a3l3 * va + a4l4 * (va + aast);
+    T Fd3 = a1l1 * (ha - una * aa) + a2l2 * aa * vna + a3l3 * h * qa2 + a4l4 * (ha + una * aa);
     f.v[0] = (Fc0 + Fd0) * length;
     f.v[1] = (Fc1 + Fd1) * length;
     f.v[2] = (Fc2 + Fd2) * length;
@@ -115,28 +132,29 @@ __device__ __forceinline__ bool roe_flux(const Consts &c, double dsx, double dsy
 }
 
 // kinetic flux-vector splitting, src/kfvs_flux.f:16-78
-__device__ __forceinline__ void kfvs_flux(const Consts &c, double dsx, double dsy, const V4 &L,
-                                          const V4 &R, V4 &f)
+template <class T = double>
+__device__ __forceinline__ void kfvs_flux(const Consts &c, T dsx, T dsy, const VT<T> &L,
+                                          const VT<T> &R, VT<T> &f)
 {
-    const double G1 = c.GAMMA1;
-    double length = sqrt(sq(dsx) + sq(dsy));
-    double ct = dsx / length, st = dsy / length;
-    double rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
-    double el = pl / G1 + 0.5 * rl * (sq(ul) + sq(vl));
-    double rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
-    double er = pr / G1 + 0.5 * rr * (sq(ur) + sq(vr));
-    double unl = ul * ct + vl * st, unr = ur * ct + vr * st;
-    double vnl = -ul * st + vl * ct, vnr = -ur * st + vr * ct;
-    double betal = 0.5 * rl / pl, sl = unl * sqrt(betal);
-    double Al = 0.5 * (1.0 + erf(sl));
-    double Bl = 0.5 * exp(-sq(sl)) / sqrt(c.M_PI_ * betal);
-    double betar = 0.5 * rr / pr, sr = unr * sqrt(betar);
-    double Ar = 0.5 * (1.0 - erf(sr));
-    double Br = 0.5 * exp(-sq(sr)) / sqrt(c.M_PI_ * betar);
-    double F0 = rl * (unl * Al + Bl) + rr * (unr * Ar - Br);
-    double F1 = ((pl + rl * sq(unl)) * Al + rl * unl * Bl) + ((pr + rr * sq(unr)) * Ar - rr * unr * Br);
-    double F2 = rl * (unl * Al + Bl) * vnl + rr * (unr * Ar - Br) * vnr;
-    double F3 = ((el + pl) * unl * Al + (el + 0.5 * pl) * Bl) + ((er + pr) * unr * Ar - (er + 0.5 * pr) * Br);
+    const T G1 = T(c.GAMMA1), h = T(0.5), PI = T(c.M_PI_);
+    T length = sqrt(dsx * dsx + dsy * dsy);
+    T ct = dsx / length, st = dsy / length;
+    T rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
+    T el = pl / G1 + h * rl * (ul * ul + vl * vl);
+    T rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
+    T er = pr / G1 + h * rr * (ur * ur + vr * vr);
+    T unl = ul * ct + vl * st, unr = ur * ct + vr * st;
+    T vnl = -ul * st + vl * ct, vnr = -ur * st + vr * ct;
+    T betal = h * rl / pl, sl = unl * sqrt(betal);
+    T Al = h * (T(1) + erf(sl));
+    T Bl = h * exp(-(sl * sl)) / sqrt(PI * betal);
+    T betar = h * rr / pr, sr = unr * sqrt(betar);
+    T Ar = h * (T(1) - erf(sr));
+    T Br = h * exp(-(sr * sr)) / sqrt(PI * betar);
+    T F0 = rl * (unl * Al + Bl) + rr * (unr * Ar - Br);
+    T F1 = ((pl + rl * (unl * unl)) * Al + rl * unl * Bl) + ((pr + rr * (unr * unr)) * Ar - rr * unr * Br);
+    T F2 = rl * (unl * Al + Bl) * vnl + rr * (unr * Ar - Br) * vnr;
+    T F3 = ((el + pl) * unl * Al + (el + h * pl) * Bl) + ((er + pr) * unr * Ar - (er + h * pr) * Br);
     f.v[0] = F0 * length;
     f.v[1] = (ct * F1 - st * F2) * length;
     f.v[2] = (st * F1 + ct * F2) * length;
@@ -145,67 +163,67 @@ __device__ __forceinline__ void kfvs_flux(const Consts &c, double dsx, double ds
 
 // HLLC (Toro ch. 10), src/hllc_flux.f:18-154.  The reference adds the
 // edge-frame flux to the x-y momentum residual; ROT=1 rotates it back.
-template <int ROT>
-__device__ __forceinline__ void hllc_flux(const Consts &c, double dsx, double dsy, const V4 &L,
-                                          const V4 &R, V4 &f)
+template <int ROT, class T = double>
+__device__ __forceinline__ void hllc_flux(const Consts &c, T dsx, T dsy, const VT<T> &L,
+                                          const VT<T> &R, VT<T> &f)
 {
-    const double gamma = c.GAMMA, G1 = c.GAMMA1;
-    double length = sqrt(sq(dsx) + sq(dsy));
-    double ct = dsx / length, st = dsy / length;
-    double rl = L.v[0], ul = L.v[1] * ct + L.v[2] * st, vl = -L.v[1] * st + L.v[2] * ct, pl = L.v[3];
-    double al = sqrt(gamma * pl / rl), El = pl / G1 + 0.5 * rl * (sq(ul) + sq(vl));
-    double rr = R.v[0], ur = R.v[1] * ct + R.v[2] * st, vr = -R.v[1] * st + R.v[2] * ct, pr = R.v[3];
-    double ar = sqrt(gamma * pr / rr), Er = pr / G1 + 0.5 * rr * (sq(ur) + sq(vr));
-    double rbar = 0.5 * (rl + rr), abar = 0.5 * (al + ar);
-    double Ppv = 0.5 * (pl + pr - (ur - ul) * rbar * abar);
-    double Upv = 0.5 * (ul + ur - (pr - pl) / (rbar * abar));
-    double prmax = (pl > pr) ? pl : pr, prmin = (pl > pr) ? pr : pl;
-    double Q = prmax / prmin, pstar, ustar;
-    if (Q < 2.0 && prmin < Ppv && Ppv < prmax) {
+    const T gamma = T(c.GAMMA), G1 = T(c.GAMMA1), h = T(0.5), one = T(1), two = T(2);
+    T length = sqrt(dsx * dsx + dsy * dsy);
+    T ct = dsx / length, st = dsy / length;
+    T rl = L.v[0], ul = L.v[1] * ct + L.v[2] * st, vl = -L.v[1] * st + L.v[2] * ct, pl = L.v[3];
+    T al = sqrt(gamma * pl / rl), El = pl / G1 + h * rl * (ul * ul + vl * vl);
+    T rr = R.v[0], ur = R.v[1] * ct + R.v[2] * st, vr = -R.v[1] * st + R.v[2] * ct, pr = R.v[3];
+    T ar = sqrt(gamma * pr / rr), Er = pr / G1 + h * rr * (ur * ur + vr * vr);
+    T rbar = h * (rl + rr), abar = h * (al + ar);
+    T Ppv = h * (pl + pr - (ur - ul) * rbar * abar);
+    T Upv = h * (ul + ur - (pr - pl) / (rbar * abar));
+    T prmax = (pl > pr) ? pl : pr, prmin = (pl > pr) ? pr : pl;
+    T Q = prmax / prmin, pstar, ustar;
+    if (Q < two && prmin < Ppv && Ppv < prmax) {
         pstar = Ppv; ustar = Upv;
     } else if (Ppv < prmin) {
-        double z = 0.5 * (gamma - 1.0) / gamma;
-        double a1 = al + ar - 0.5 * (gamma - 1.0) * (ur - ul);
-        double a2 = al / pow(pl, z) + ar / pow(pr, z);
-        pstar = pow(a1 / a2, 1.0 / z);
-        double Plr = pow(pl / pr, z);
-        double b1 = Plr * ul / al + ur / ar + 2.0 * (Plr - 1.0) / (gamma - 1.0);
-        double b2 = Plr / al + 1.0 / ar;
+        T z = h * (gamma - one) / gamma;
+        T a1 = al + ar - h * (gamma - one) * (ur - ul);
+        T a2 = al / pow(pl, z) + ar / pow(pr, z);
+        pstar = pow(a1 / a2, one / z);
+        T Plr = pow(pl / pr, z);
+        T b1 = Plr * ul / al + ur / ar + two * (Plr - one) / (gamma - one);
+        T b2 = Plr / al + one / ar;
         ustar = b1 / b2;
     } else {
-        double po = fmax(0.0, Ppv);
-        double Ml = 2.0 / (gamma + 1.0) / rl, Nl = (gamma - 1.0) * pl / (gamma + 1.0);
-        double gl = sqrt(Ml / (po + Nl));
-        double Mr = 2.0 / (gamma + 1.0) / rr, Nr = (gamma - 1.0) * pr / (gamma + 1.0);
-        double gr = sqrt(Mr / (po + Nr));
+        T po = fmax(T(0), Ppv);
+        T Ml = two / (gamma + one) / rl, Nl = (gamma - one) * pl / (gamma + one);
+        T gl = sqrt(Ml / (po + Nl));
+        T Mr = two / (gamma + one) / rr, Nr = (gamma - one) * pr / (gamma + one);
+        T gr = sqrt(Mr / (po + Nr));
         pstar = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
-        ustar = 0.5 * (ul + ur + (pstar - pr) * gr - (pstar - pl) * gl);
+        ustar = h * (ul + ur + (pstar - pr) * gr - (pstar - pl) * gl);
     }
-    double ql = (pstar <= pl) ? 1.0 : sqrt(1.0 + 0.5 * (gamma + 1.0) * (pstar / pl - 1.0) / gamma);
-    double qr = (pstar <= pr) ? 1.0 : sqrt(1.0 + 0.5 * (gamma + 1.0) * (pstar / pr - 1.0) / gamma);
-    double Sl = ul - al * ql, Ss = ustar, Sr = ur + ar * qr;
-    double F[4];
-    if (0.0 <= Sl) {
-        F[0] = rl * ul; F[1] = pl + rl * sq(ul); F[2] = rl * ul * vl; F[3] = (El + pl) * ul;
-    } else if (Sl <= 0.0 && 0.0 <= Ss) {
-        double fact = rl * (Sl - ul) / (Sl - Ss);
-        double t1 = El / rl + (Ss - ul) * (Ss + pl / rl / (Sl - ul));
+    T ql = (pstar <= pl) ? one : sqrt(one + h * (gamma + one) * (pstar / pl - one) / gamma);
+    T qr = (pstar <= pr) ? one : sqrt(one + h * (gamma + one) * (pstar / pr - one) / gamma);
+    T Sl = ul - al * ql, Ss = ustar, Sr = ur + ar * qr;
+    T F[4];
+    if (T(0) <= Sl) {
+        F[0] = rl * ul; F[1] = pl + rl * (ul * ul); F[2] = rl * ul * vl; F[3] = (El + pl) * ul;
+    } else if (Sl <= T(0) && T(0) <= Ss) {
+        T fact = rl * (Sl - ul) / (Sl - Ss);
+        T t1 = El / rl + (Ss - ul) * (Ss + pl / rl / (Sl - ul));
         F[0] = rl * ul + Sl * (fact - rl);
-        F[1] = (pl + rl * sq(ul)) + Sl * (fact * Ss - rl * ul);
+        F[1] = (pl + rl * (ul * ul)) + Sl * (fact * Ss - rl * ul);
         F[2] = rl * ul * vl + Sl * (fact * vl - rl * vl);
         F[3] = (El + pl) * ul + Sl * (fact * t1 - El);
-    } else if (Ss <= 0.0 && 0.0 <= Sr) {
-        double fact = rr * (Sr - ur) / (Sr - Ss);
-        double t1 = Er / rr + (Ss - ur) * (Ss + pr / rr / (Sr - ur));
+    } else if (Ss <= T(0) && T(0) <= Sr) {
+        T fact = rr * (Sr - ur) / (Sr - Ss);
+        T t1 = Er / rr + (Ss - ur) * (Ss + pr / rr / (Sr - ur));
         F[0] = rr * ur + Sr * (fact - rr);
-        F[1] = (pr + rr * sq(ur)) + Sr * (fact * Ss - rr * ur);
+        F[1] = (pr + rr * (ur * ur)) + Sr * (fact * Ss - rr * ur);
         F[2] = rr * ur * vr + Sr * (fact * vr - rr * vr);
         F[3] = (Er + pr) * ur + Sr * (fact * t1 - Er);
     } else {
-        F[0] = rr * ur; F[1] = pr + rr * sq(ur); F[2] = rr * ur * vr; F[3] = (Er + pr) * ur;
+        F[0] = rr * ur; F[1] = pr + rr * (ur * ur); F[2] = rr * ur * vr; F[3] = (Er + pr) * ur;
     }
     if (ROT) {
-        double fx = ct * F[1] - st * F[2], fy = st * F[1] + ct * F[2];
+        T fx = ct * F[1] - st * F[2], fy = st * F[1] + ct * F[2];
         F[1] = fx; F[2] = fy;
     }
 #pragma unroll

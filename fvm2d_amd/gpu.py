@@ -70,6 +70,8 @@ def lib():
     L.fvm_gpu_destroy.argtypes = [vp]
     L.fvm_gpu_set_state.restype = ci
     L.fvm_gpu_set_state.argtypes = [vp, vp, vp]
+    L.fvm_gpu_wall_dist.restype = ci
+    L.fvm_gpu_wall_dist.argtypes = [ci, vp, vp, ci, vp, ci, vp, vp]
     L.fvm_gpu_tune.restype = ci
     L.fvm_gpu_tune.argtypes = [vp, ci, ci, ci]
     L.fvm_gpu_set_cl.restype = ci
@@ -83,7 +85,7 @@ def lib():
     L.fvm_gpu_time_kernels.restype = ci
     L.fvm_gpu_time_kernels.argtypes = [vp, ci, vp, ctypes.POINTER(ctypes.c_char_p), ci]
     L.fvm_patch_plan.restype = ci
-    L.fvm_patch_plan.argtypes = [ctypes.POINTER(Mesh), ci, ci, vp]
+    L.fvm_patch_plan.argtypes = [ctypes.POINTER(Mesh), ci, ci, ci, vp]
     L.fvm_gpu_footprint.restype = ctypes.c_longlong
     L.fvm_gpu_footprint.argtypes = [vp]
     L.fvm_gpu_footprint_estimate.restype = ctypes.c_longlong

@@ -90,8 +90,22 @@ def _p(a: np.ndarray):
 
 
 def geometric(coord, elem, ptype, cell_type: int = 1, iflow: int = 1,
-              sa_compat: int = 1) -> Geometry:
-    """Dual-mesh preprocessing (src/geometric.f:2-63) through fvm_geometric."""
+              sa_compat: int = 1, wall_distance: str = "host") -> Geometry:
+    """Dual-mesh preprocessing (src/geometric.f:2-63) through fvm_geometric.
+    For SA-RANS (iflow 3) the wall distance comes from the host loop or, with
+    ``wall_distance="gpu"``, from fvm_gpu_wall_dist (same values)."""
+    if iflow == 3 and wall_distance == "gpu":
+        geo = geometric(coord, elem, ptype, cell_type, 1, sa_compat)
+        wd = np.zeros(geo.dims["np"])
+        wd1 = np.zeros(max(geo.dims["nsp"], 1))
+        rc = gpu.lib().fvm_gpu_wall_dist(geo.dims["np"], _p(geo.coord), _p(geo.ptype),
+                                         geo.dims["netot"], _p(geo.edge), sa_compat, _p(wd),
+                                         _p(wd1))
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_gpu_wall_dist")
+        geo.wd = wd
+        geo.wd1 = wd1[:geo.dims["nsp"]].copy()
+        return geo
     L = gpu.lib()
     coord = np.ascontiguousarray(coord, dtype=np.float64).reshape(-1, 2)
     elem = np.ascontiguousarray(elem, dtype=np.int32).reshape(-1, 3).copy()
@@ -141,12 +155,12 @@ def mesh_struct(g: Geometry, keep: list) -> Mesh:
                 drmin=_p(g.drmin), wd=_p(g.wd))
 
 
-def patch_plan(geo: Geometry, iflow: int = 1, patch_size: int = 0) -> dict:
+def patch_plan(geo: Geometry, iflow: int = 1, patch_size: int = 0, lds_limit: int = 0) -> dict:
     """Layout of the LDS-tiled stage kernel for a mesh (host only)."""
     keep: list = []
     m = mesh_struct(geo, keep)
     st = np.zeros(8, dtype=np.int64)
-    rc = gpu.lib().fvm_patch_plan(ctypes.byref(m), iflow, patch_size, _p(st))
+    rc = gpu.lib().fvm_patch_plan(ctypes.byref(m), iflow, patch_size, lds_limit, _p(st))
     names = ("patches", "max_patch", "max_local", "max_triangles", "max_edges", "lds_bytes",
              "halo_total", "failed_check")
     out = dict(zip(names, (int(v) for v in st)))

@@ -95,6 +95,14 @@ int fvm_geometric(int np, int nt, const double *coord, int *elem, const int *pty
                   double *elarea, double *cvarea, double *mcarea, double *drmin,
                   double *wd, double *wd1, int *beindx, int *spts, int *dims);
 
+/* Wall distance on the current GPU, replacing wall_dist(...)
+ * (src/wall_dist.f:3-95), bit for bit: wd(np) for every point (1 on solid
+ * points, as the reference leaves it) and, if wd1 is not NULL, wd1(nsp) the
+ * distance of the nearest non-solid neighbour of each solid point.  edge is
+ * the (2,netot) array fvm_geometric returns; sa_compat as in fvm_params. */
+int fvm_gpu_wall_dist(int np, const double *coord, const int *ptype, int netot, const int *edge,
+                      int sa_compat, double *wd, double *wd1);
+
 /* Uploads the mesh, builds the device layout and sets the free-stream state
  * (src/initialize.f:14-90 with Start 1).  One context = one solve, on the
  * current HIP device, with its own stream. */
@@ -136,19 +144,21 @@ int fvm_gpu_iterate(fvm_ctx *ctx, int n, double *hist, int *ndone, int *err_inde
 int fvm_gpu_time_kernels(fvm_ctx *ctx, int n, double *ms, const char **names, int nk);
 
 /* Tuning knobs (no counterpart in the reference; used by the autotuner):
- * variant 1 = edge-parallel flux into a per-edge buffer + vertex gathers
- * (three kernels per RK stage); variant 2 = slot-parallel, LDS-staged
- * gradient and fused flux+update (two kernels per stage).  Both keep the
- * reference's summation order.  vertices_per_block in [16, 256];
- * xcd_remap gives each XCD a contiguous range of vertex blocks. */
+ * variant 3 (default) = the LDS-tiled patch stage kernel, one launch per RK
+ * stage; variant 1 = edge-parallel flux into a per-edge buffer + vertex
+ * gathers (three kernels per stage).  Both keep the reference's summation
+ * order.  vertices_per_block in [16, 256] applies to variant 1; xcd_remap
+ * gives each XCD a contiguous range of patches / vertex blocks. */
 int fvm_gpu_tune(fvm_ctx *ctx, int variant, int vertices_per_block, int xcd_remap);
 
 /* Host only (no GPU needed): plans the device layout of a mesh for the
  * LDS-tiled stage kernel and checks every index the kernels will use.
- * stats[8]: patches, max patch vertices, max local vertices (patch and its
- * two rings), max local triangles, max local edges, LDS bytes per
- * workgroup, total halo vertices, failed check (0 = none). */
-int fvm_patch_plan(const fvm_mesh *mesh, int iflow, int patch_size, long long *stats);
+ * lds_limit: dynamic LDS bytes a workgroup may use (0 = 58 KiB; contexts use
+ * the device's own limit).  stats[8]: patches, max patch vertices, max local
+ * vertices (patch and its two rings), max local triangles, max local edges,
+ * LDS bytes per workgroup, total halo vertices, failed check (0 = none). */
+int fvm_patch_plan(const fvm_mesh *mesh, int iflow, int patch_size, int lds_limit,
+                   long long *stats);
 
 /* Bytes of device memory the context holds (admission by HBM footprint). */
 long long fvm_gpu_footprint(const fvm_ctx *ctx);

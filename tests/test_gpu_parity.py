@@ -182,12 +182,12 @@ def test_workload_grid_other_fluxes_match_oracle(iflux):
 @pytest.mark.parametrize("name", ["euler_roe_transonic_65x17", "euler_kfvs_lim_129x33",
                                   "laminar_roe_97x25", "rans_roe_97x25", "euler_roe_barth_65x17",
                                   "euler_roe_vortex_65x17"])
-@pytest.mark.parametrize("shape", [(1, 64, 0, 0), (2, 16, 0, 0), (2, 64, 1, 0), (2, 256, 1, 0),
-                                   (3, 64, 1, 16), (3, 64, 0, 64), (3, 64, 1, 256)])
+@pytest.mark.parametrize("shape", [(1, 64, 0, 0), (1, 256, 1, 0), (3, 64, 1, 16), (3, 64, 0, 64),
+                                   (3, 64, 1, 256)])
 def test_kernel_variants_bit_identical(name, shape):
-    """Every kernel layout (edge-flux buffer, slot-parallel, patch stage
-    kernel) and every launch shape adds the same terms in the same order:
-    histories and states agree bit for bit."""
+    """Both kernel layouts (edge-flux buffer, patch stage kernel) and every
+    launch shape add the same terms in the same order: histories and states
+    agree bit for bit."""
     out = []
     for sh in [(3, 64, 1, 128), shape]:
         with solver_for(name, patch_size=sh[3]) as s:

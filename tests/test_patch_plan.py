@@ -15,7 +15,7 @@ def test_plan_valid(cell, iflow, ps):
     st = patch_plan(geo, iflow, ps)
     assert st["failed_check"] == 0
     assert st["max_patch"] <= ps
-    assert st["lds_bytes"] <= 56 * 1024
+    assert st["lds_bytes"] <= 58 * 1024
     assert st["patches"] * st["max_patch"] >= geo.dims["np"]
 
 
@@ -24,3 +24,5 @@ def test_plan_workload_grid():
     geo = geometric(g.coord, g.elem, g.ptype)
     st = patch_plan(geo, 1, 64)
     assert st["failed_check"] == 0 and st["patches"] == 66048 // 64
+    big = patch_plan(geo, 1, 256, lds_limit=150 * 1024)
+    assert big["failed_check"] == 0 and big["max_patch"] == 256
