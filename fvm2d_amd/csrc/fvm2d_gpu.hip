@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fvm2d_gpu.h"
@@ -41,6 +42,9 @@ using namespace fvm;
 namespace {
 
 constexpr int kBlock = 256;
+// waves per SIMD the stage kernel is compiled for (one workgroup of
+// kBlock threads puts one wave on each SIMD of a CU)
+constexpr int kPatchWavesEuler = 4, kPatchWavesVisc = 3;
 constexpr int kHistCap = 4096;      // history rows per fvm_gpu_iterate chunk
 constexpr int kGraphIters = 8;      // iterations captured per graph
 
@@ -79,10 +83,9 @@ struct Dev {
 
 // save_old + time_step, src/save_old.f:11-15, src/time_step.f:13-28
 template <bool VISC>
-__global__ void __launch_bounds__(kBlock) k_tstep(Dev d, Consts c)
+__device__ __forceinline__ void tstep_body(const Dev &d, const Consts &c, int v)
 {
     if (d.S->done) return;
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= d.np) return;
     double4 p = d.prim[v];
     d.prim_old[v] = p;
@@ -95,6 +98,12 @@ __global__ void __launch_bounds__(kBlock) k_tstep(Dev d, Consts c)
     } else {
         d.dt[v] = c.cfl * dr / (q + a);
     }
+}
+
+template <bool VISC>
+__global__ void __launch_bounds__(kBlock) k_tstep(Dev d, Consts c)
+{
+    tstep_body<VISC>(d, c, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Sutherland viscosity and total viscosity, src/residu.f:23-26
@@ -258,22 +267,49 @@ __device__ __forceinline__ Red red_join(Red a, Red b)
     return r;
 }
 
+// Block reductions: a butterfly over the 64 lanes of each wavefront (every
+// lane ends with the wavefront's total), then the wavefront totals in order.
+// The pattern is fixed, so results repeat bit for bit; every thread returns
+// the total.
+__device__ __forceinline__ Red shfl_red(Red a, int o)
+{
+    return Red{__shfl_xor(a.s, o), __shfl_xor(a.m, o), __shfl_xor(a.i, o)};
+}
+
+template <int NT>
 __device__ Red block_reduce(Red x)
 {
-    __shared__ double ss[kBlock], sm[kBlock];
-    __shared__ int si[kBlock];
-    int t = threadIdx.x;
-    ss[t] = x.s; sm[t] = x.m; si[t] = x.i;
+    __shared__ double ss[NT / 64], sm[NT / 64];
+    __shared__ int si[NT / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = red_join(x, shfl_red(x, o));
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[w] = x.s; sm[w] = x.m; si[w] = x.i; }
     __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (t < w) {
-            Red a{ss[t], sm[t], si[t]}, b{ss[t + w], sm[t + w], si[t + w]};
-            Red r = red_join(a, b);
-            ss[t] = r.s; sm[t] = r.m; si[t] = r.i;
-        }
-        __syncthreads();
+    Red r{ss[0], sm[0], si[0]};
+#pragma unroll
+    for (int k = 1; k < NT / 64; ++k) r = red_join(r, Red{ss[k], sm[k], si[k]});
+    __syncthreads();
+    return r;
+}
+
+template <int NT>
+__device__ double2 block_sum2(double2 x)
+{
+    __shared__ double2 s2[NT / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        x.x = x.x + __shfl_xor(x.x, o);
+        x.y = x.y + __shfl_xor(x.y, o);
     }
-    return Red{ss[0], sm[0], si[0]};
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) s2[w] = x;
+    __syncthreads();
+    double2 r = s2[0];
+#pragma unroll
+    for (int k = 1; k < NT / 64; ++k) { r.x = r.x + s2[k].x; r.y = r.y + s2[k].y; }
+    __syncthreads();
+    return r;
 }
 
 // RK stage, src/update.f:17-35 (+ src/convert.f).  LAST: the third stage,
@@ -313,7 +349,7 @@ __global__ void __launch_bounds__(kBlock) k_update(Dev d, Consts c, DevState *S)
         }
     }
     if (LAST) {
-        Red r = block_reduce(red);
+        Red r = block_reduce<kBlock>(red);
         if (threadIdx.x == 0) {
             d.partial[3 * blockIdx.x] = r.s;
             d.partial[3 * blockIdx.x + 1] = r.m;
@@ -337,10 +373,9 @@ __global__ void __launch_bounds__(kBlock) k_res(Dev d, double4 *res)
 // and the semi-implicit update (update_sa), then nu_tilde = 0 on the wall
 // and the positivity check.  Reads nut, writes nut_new.
 template <bool COMPAT>
-__global__ void __launch_bounds__(kBlock) k_sa(Dev d, Consts c, DevState *S)
+__device__ __forceinline__ void sa_body(const Dev &d, const Consts &c, DevState *S, int v)
 {
     if (S->done || S->err_code) return;
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < d.np) {
         const double *nut = d.nut;
         double divf = 0.0;
@@ -413,6 +448,12 @@ __global__ void __launch_bounds__(kBlock) k_sa(Dev d, Consts c, DevState *S)
     }
 }
 
+template <bool COMPAT>
+__global__ void __launch_bounds__(kBlock) k_sa(Dev d, Consts c, DevState *S)
+{
+    sa_body<COMPAT>(d, c, S, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int n, const DevState *S)
 {
     if (S->done) return;
@@ -422,58 +463,56 @@ __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int
 
 // Monitors and the stop test of one iteration: conres (src/conres.f:11-31),
 // clcd (src/clcd.f:15-46), the RES.DAT row (src/main.f:63) and the loop
-// condition (src/main.f:42, :67-75).  One workgroup; fixed reduction order.
-__global__ void __launch_bounds__(kBlock) k_final(Dev d, Consts c, DevState *S)
+// condition (src/main.f:42, :67-75).  One wide workgroup, so the partials and
+// the solid edges are each read in one pass; fixed reduction order.  Without
+// viscosity the stress terms are zero and are left out.
+constexpr int kFinal = 1024;
+
+template <bool VISC>
+__device__ __forceinline__ void final_body(const Dev &d, const Consts &c, DevState *S)
 {
     if (S->done) return;
-    __shared__ int s_err;
-    if (threadIdx.x == 0) s_err = S->err_code;
-    __syncthreads();
-    if (s_err) {
-        if (threadIdx.x == 0) { S->status = s_err; S->done = 1; }
+    const int err = S->err_code;
+    const int t = threadIdx.x;
+    if (err) {
+        if (t == 0) { S->status = err; S->done = 1; }
         return;
     }
-    int t = threadIdx.x;
     Red acc{0.0, 0.0, 0};
-    for (int k = t; k < d.nparts; k += kBlock)
+#pragma unroll 2
+    for (int k = t; k < d.nparts; k += kFinal)
         acc = red_join(acc, Red{d.partial[3 * k], d.partial[3 * k + 1], (int)d.partial[3 * k + 2]});
-    Red tot = block_reduce(acc);
-    __syncthreads();
     // forces on the solid edges
     double xf = 0.0, yf = 0.0;
-    for (int b = t; b < d.nsw; b += kBlock) {
+#pragma unroll 1
+    for (int b = t; b < d.nsw; b += kFinal) {
         int2 e = d.bnode[b];
         double2 x1 = d.coord[e.x], x2 = d.coord[e.y];
         double sx = x2.y - x1.y, sy = -(x2.x - x1.x);
-        double mu1 = d.mu[e.x], mu2 = d.mu[e.y];
-        double4 qx1 = d.qx[e.x], qy1 = d.qy[e.x], qx2 = d.qx[e.y], qy2 = d.qy[e.y];
         double p1 = d.prim[e.x].w, p2 = d.prim[e.y].w;
-        double txx1 = 2.0 / 3.0 * mu1 * (2.0 * qx1.y - qy1.z);
-        double txy1 = mu1 * (qy1.y + qx1.z);
-        double tyy1 = 2.0 / 3.0 * mu1 * (2.0 * qy1.z - qx1.y);
-        double txx2 = 2.0 / 3.0 * mu2 * (2.0 * qx2.y - qy2.z);
-        double txy2 = mu2 * (qy2.y + qx2.z);
-        double tyy2 = 2.0 / 3.0 * mu2 * (2.0 * qy2.z - qx2.y);
-        double p = 0.5 * (p1 + p2), txx = 0.5 * (txx1 + txx2), txy = 0.5 * (txy1 + txy2),
-               tyy = 0.5 * (tyy1 + tyy2);
-        xf = xf + p * sx - txx * sx - txy * sy;
-        yf = yf + p * sy - txy * sx - tyy * sy;
-    }
-    Red fx = block_reduce(Red{xf, 0.0, 0});
-    __syncthreads();
-    Red fy = block_reduce(Red{yf, 0.0, 0});
-    double qdyn = 0.5 * c.r_inf * sq(c.q_inf);
-    double cl = (-sin(c.aoa) * fx.s + cos(c.aoa) * fy.s) / qdyn;
-    double cd = (cos(c.aoa) * fx.s + sin(c.aoa) * fy.s) / qdyn;
-    if (c.vortex == 1 && c.mach_inf < 1.0) {
-        // far-field states of the next iteration, with this iteration's cl
-        for (int b = t; b < d.nff; b += kBlock) {
-            int2 e = d.bnode[d.nsw + b];
-            double u[4];
-            farfield_state(c, cl, d.coord[e.x], d.coord[e.y], u);
-            d.uout[b] = make_double4(u[0], u[1], u[2], u[3]);
+        double p = 0.5 * (p1 + p2);
+        if (VISC) {
+            double mu1 = d.mu[e.x], mu2 = d.mu[e.y];
+            double4 qx1 = d.qx[e.x], qy1 = d.qy[e.x], qx2 = d.qx[e.y], qy2 = d.qy[e.y];
+            double txx1 = 2.0 / 3.0 * mu1 * (2.0 * qx1.y - qy1.z);
+            double txy1 = mu1 * (qy1.y + qx1.z);
+            double tyy1 = 2.0 / 3.0 * mu1 * (2.0 * qy1.z - qx1.y);
+            double txx2 = 2.0 / 3.0 * mu2 * (2.0 * qx2.y - qy2.z);
+            double txy2 = mu2 * (qy2.y + qx2.z);
+            double tyy2 = 2.0 / 3.0 * mu2 * (2.0 * qy2.z - qx2.y);
+            double txx = 0.5 * (txx1 + txx2), txy = 0.5 * (txy1 + txy2), tyy = 0.5 * (tyy1 + tyy2);
+            xf = xf + p * sx - txx * sx - txy * sy;
+            yf = yf + p * sy - txy * sx - tyy * sy;
+        } else {
+            xf = xf + p * sx;
+            yf = yf + p * sy;
         }
     }
+    const Red tot = block_reduce<kFinal>(acc);
+    const double2 f = block_sum2<kFinal>(make_double2(xf, yf));
+    double qdyn = 0.5 * c.r_inf * sq(c.q_inf);
+    double cl = (-c.sin_aoa * f.x + c.cos_aoa * f.y) / qdyn;
+    double cd = (c.cos_aoa * f.x + c.sin_aoa * f.y) / qdyn;
     if (t != 0) return;
     double fres = sqrt(tot.s / c.np);
     if (S->iter == c.iterlast) {
@@ -495,6 +534,12 @@ __global__ void __launch_bounds__(kBlock) k_final(Dev d, Consts c, DevState *S)
     }
 }
 
+template <bool VISC>
+__global__ void __launch_bounds__(kFinal) k_final(Dev d, Consts c, DevState *S)
+{
+    final_body<VISC>(d, c, S);
+}
+
 
 // ------------------------------------------------- launch helpers
 
@@ -509,26 +554,47 @@ __device__ __forceinline__ int logical_block(int nb, int xcd)
 }
 
 // ------------------------------------------------- patch stage kernel
+
+// Green-Gauss term of one triangle, src/green_gauss.f:97-127: for each
+// variable, sum over the corners of q_k (dy, -dx) of the opposite side.
+__device__ __forceinline__ void tri_gg(const double2 *s_coord, const double4 *s_prim, short4 tr,
+                                       V4 &p1, V4 &p2, V4 &p3, double bx[3], double by[3],
+                                       double dxt[4], double dyt[4])
+{
+    double2 x1 = s_coord[tr.x], x2 = s_coord[tr.y], x3 = s_coord[tr.z];
+    double4 a1 = s_prim[tr.x], a2 = s_prim[tr.y], a3 = s_prim[tr.z];
+    p1 = V4{{a1.x, a1.y, a1.z, a1.w}}; p2 = V4{{a2.x, a2.y, a2.z, a2.w}}; p3 = V4{{a3.x, a3.y, a3.z, a3.w}};
+    bx[0] = x2.y - x3.y; bx[1] = x3.y - x1.y; bx[2] = x1.y - x2.y;
+    by[0] = x3.x - x2.x; by[1] = x1.x - x3.x; by[2] = x2.x - x1.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        dxt[i] = p1.v[i] * bx[0] + p2.v[i] * bx[1] + p3.v[i] * bx[2];
+        dyt[i] = p1.v[i] * by[0] + p2.v[i] * by[1] + p3.v[i] * by[2];
+    }
+}
+
 template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
-__global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, PatchArgs pa,
-                                                  int irk, const double4 *__restrict__ pin,
-                                                  double4 *__restrict__ pout, double4 *res_out)
+__device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevState *S,
+                                           const PatchArgs &pa, const int b, const int irk,
+                                           const double4 *__restrict__ pin,
+                                           double4 *__restrict__ pout, double4 *res_out)
 {
     if (S->done || S->err_code) return;
     extern __shared__ double4 smem4[];
     double *smem = reinterpret_cast<double *>(smem4);
-    const int b = logical_block(pa.npatch, d.xcd);
     const int *h = pa.hdr + kPatchHdr * b;
     const int v0 = h[PH_V0], NP = h[PH_NP], NL = h[PH_NL], NG = h[PH_NG];
     const int NLa = (NL + 1) & ~1;               // keeps every LDS array 16-byte aligned
     const int NT = h[PH_NTRI], NE = h[PH_NEDGE], NB = h[PH_NBND];
+    const int NGTL = h[PH_NGTL], NPSL = h[PH_NPSL];
     const int *halo = pa.halo + h[PH_HALO];
     const short4 *ltri = pa.ltri + h[PH_TRI];
-    const int *gtri = pa.gtri + h[PH_TRI];
+    const double *larea = pa.larea + h[PH_TRI];
     const int *gptr = pa.gptr + h[PH_GPTR];
     const short *gtl = pa.gtl + h[PH_GTL];
     const short2 *ledge = pa.ledge + h[PH_EDGE];
     const int *ledge_id = pa.ledge_id + h[PH_EDGE];
+    const double2 *ledge_ds = pa.ledge_ds + h[PH_EDGE];
     const short2 *lbnd = pa.lbnd + h[PH_BND];
     const int *lbnd_id = pa.lbnd_id + h[PH_BND];
     const int *pptr = pa.pptr + h[PH_PPTR];
@@ -544,7 +610,16 @@ __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, 
     double4 *s_gx = reinterpret_cast<double4 *>(s_nut + (VISC ? NLa : 0));
     double4 *s_gy = s_gx + NG;
     double *s_work = reinterpret_cast<double *>(s_gy + NG);
-    constexpr int TW = VISC ? 14 : 8;            // doubles per local triangle
+    // work area: viscous flow stages per-triangle terms (TW doubles each);
+    // inviscid flow keeps the local triangles there and recomputes each
+    // triangle's Green-Gauss term per vertex (same operands, same bits)
+    constexpr int TW = VISC ? 14 : 1;
+    short4 *s_ltri = reinterpret_cast<short4 *>(s_work);
+    const int WB = max(NT * TW * 8, (NE + 2 * NB) * 32);
+    int *s_gptr = reinterpret_cast<int *>(reinterpret_cast<char *>(s_work) + WB);
+    int *s_pptr = s_gptr + NG + 1;
+    short *s_gtl = reinterpret_cast<short *>(s_pptr + NP + 1);
+    short *s_psl = s_gtl + NGTL;
 
     // 1. stage state of P, H1, H2
     for (int i = t; i < NL; i += kBlock) {
@@ -560,62 +635,72 @@ __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, 
             if (i < NP) { d.mul[gv] = ml; d.mu[gv] = m; }
         }
     }
+    for (int i = t; i <= NG; i += kBlock) s_gptr[i] = gptr[i];
+    for (int i = t; i <= NP; i += kBlock) s_pptr[i] = pptr[i];
+    for (int i = t; i < NGTL; i += kBlock) s_gtl[i] = gtl[i];
+    for (int i = t; i < NPSL; i += kBlock) s_psl[i] = pslot[i];
+    if (!VISC)
+        for (int i = t; i < NT; i += kBlock) s_ltri[i] = ltri[i];
     __syncthreads();
 
-    // 2. triangles touching G
-    for (int k = t; k < NT; k += kBlock) {
-        short4 tr = ltri[k];
-        double2 x1 = s_coord[tr.x], x2 = s_coord[tr.y], x3 = s_coord[tr.z];
-        double4 a1 = s_prim[tr.x], a2 = s_prim[tr.y], a3 = s_prim[tr.z];
-        V4 p1{{a1.x, a1.y, a1.z, a1.w}}, p2{{a2.x, a2.y, a2.z, a2.w}}, p3{{a3.x, a3.y, a3.z, a3.w}};
-        double bx[3] = {x2.y - x3.y, x3.y - x1.y, x1.y - x2.y};
-        double by[3] = {x3.x - x2.x, x1.x - x3.x, x2.x - x1.x};
-        double dxt[4], dyt[4];
-        double *w = s_work + TW * k;
+    // 2. viscous flow: triangles touching G, each once
+    if (VISC) {
+        for (int k = t; k < NT; k += kBlock) {
+            short4 tr = ltri[k];
+            V4 p1, p2, p3;
+            double bx[3], by[3], dxt[4], dyt[4];
+            tri_gg(s_coord, s_prim, tr, p1, p2, p3, bx, by, dxt, dyt);
+            double *w = s_work + TW * k;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            dxt[i] = p1.v[i] * bx[0] + p2.v[i] * bx[1] + p3.v[i] * bx[2];
-            dyt[i] = p1.v[i] * by[0] + p2.v[i] * by[1] + p3.v[i] * by[2];
-            w[i] = dxt[i];
-            w[4 + i] = dyt[i];
+            for (int i = 0; i < 4; ++i) { w[i] = dxt[i]; w[4 + i] = dyt[i]; }
+            {
+                const double n2b3 = 2.0 / 3.0;
+                double e1 = p1.v[3] / p1.v[0], e2 = p2.v[3] / p2.v[0], e3 = p3.v[3] / p3.v[0];
+                double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
+                double eyt = e1 * by[0] + e2 * by[1] + e3 * by[2];
+                double um = (p1.v[1] + p2.v[1] + p3.v[1]) / 3.0;
+                double vm = (p1.v[2] + p2.v[2] + p3.v[2]) / 3.0;
+                double mut = (s_mu[tr.x] + s_mu[tr.y] + s_mu[tr.z]) / 3.0;
+                double mult = (s_mul[tr.x] + s_mul[tr.y] + s_mul[tr.z]) / 3.0;
+                double prim5t = (s_nut[tr.x] + s_nut[tr.y] + s_nut[tr.z]) / 3.0;
+                double rhot = (p1.v[0] + p2.v[0] + p3.v[0]) / 3.0;
+                double chi = prim5t * rhot / mult;
+                double chi3 = pow3(chi);
+                double fv1 = chi3 / (chi3 + c.Cv11);
+                double txx = n2b3 * mut * (2.0 * dxt[1] - dyt[2]);
+                double txy = mut * (dyt[1] + dxt[2]);
+                double tyy = n2b3 * mut * (2.0 * dyt[2] - dxt[1]);
+                double gpr = c.GAMMA * (mult / c.prandtl + rhot * prim5t * fv1 / c.prandtl_turb) / c.GAMMA1;
+                w[8] = txx; w[9] = txy; w[10] = tyy;
+                w[11] = txx * um + txy * vm + gpr * ext;
+                w[12] = txy * um + tyy * vm + gpr * eyt;
+                w[13] = 0.25 / larea[k];
+            }
         }
-        if (VISC) {
-            const double n2b3 = 2.0 / 3.0;
-            double e1 = p1.v[3] / p1.v[0], e2 = p2.v[3] / p2.v[0], e3 = p3.v[3] / p3.v[0];
-            double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
-            double eyt = e1 * by[0] + e2 * by[1] + e3 * by[2];
-            double um = (p1.v[1] + p2.v[1] + p3.v[1]) / 3.0;
-            double vm = (p1.v[2] + p2.v[2] + p3.v[2]) / 3.0;
-            double mut = (s_mu[tr.x] + s_mu[tr.y] + s_mu[tr.z]) / 3.0;
-            double mult = (s_mul[tr.x] + s_mul[tr.y] + s_mul[tr.z]) / 3.0;
-            double prim5t = (s_nut[tr.x] + s_nut[tr.y] + s_nut[tr.z]) / 3.0;
-            double rhot = (p1.v[0] + p2.v[0] + p3.v[0]) / 3.0;
-            double chi = prim5t * rhot / mult;
-            double chi3 = pow3(chi);
-            double fv1 = chi3 / (chi3 + c.Cv11);
-            double txx = n2b3 * mut * (2.0 * dxt[1] - dyt[2]);
-            double txy = mut * (dyt[1] + dxt[2]);
-            double tyy = n2b3 * mut * (2.0 * dyt[2] - dxt[1]);
-            double gpr = c.GAMMA * (mult / c.prandtl + rhot * prim5t * fv1 / c.prandtl_turb) / c.GAMMA1;
-            w[8] = txx; w[9] = txy; w[10] = tyy;
-            w[11] = txx * um + txy * vm + gpr * ext;
-            w[12] = txy * um + tyy * vm + gpr * eyt;
-            w[13] = 0.25 / d.elarea[gtri[k]];
-        }
+        __syncthreads();
     }
-    __syncthreads();
 
     // 3. gradients of G; viscous residual of P kept by P's own thread
     double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
     for (int g = t; g < NG; g += kBlock) {
+        const int gv = g < NP ? v0 + g : halo[g - NP];
+        const double ais = d.ais[gv];
         double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
-        const int k1 = gptr[g + 1];
-        for (int k = gptr[g]; k < k1; ++k) {
-            const int e = gtl[k];
+        const int k1 = s_gptr[g + 1];
+        for (int k = s_gptr[g]; k < k1; ++k) {
+            const int e = s_gtl[k];
+            if (!VISC) {
+                V4 p1, p2, p3;
+                double bx[3], by[3], dxt[4], dyt[4];
+                tri_gg(s_coord, s_prim, s_ltri[e >> 2], p1, p2, p3, bx, by, dxt, dyt);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) { gx[i] = gx[i] + dxt[i]; gy[i] = gy[i] + dyt[i]; }
+                continue;
+            }
             const double *w = s_work + TW * (e >> 2);
 #pragma unroll
             for (int i = 0; i < 4; ++i) { gx[i] = gx[i] + w[i]; gy[i] = gy[i] + w[4 + i]; }
-            if (VISC && g < NP) {
+            if (g < NP) {
                 short4 tr = ltri[e >> 2];
                 const int m = e & 3;
                 double2 x1 = s_coord[tr.x], x2 = s_coord[tr.y], x3 = s_coord[tr.z];
@@ -627,8 +712,6 @@ __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, 
                 rv3 = rv3 + (w[11] * bxm + w[12] * bym) * areat;
             }
         }
-        const int gv = g < NP ? v0 + g : halo[g - NP];
-        const double ais = d.ais[gv];
         double4 qx = make_double4(gx[0] * ais, gx[1] * ais, gx[2] * ais, gx[3] * ais);
         double4 qy = make_double4(gy[0] * ais, gy[1] * ais, gy[2] * ais, gy[3] * ais);
         s_gx[g] = qx;
@@ -641,8 +724,7 @@ __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, 
     int bad = INT_MAX;
     for (int k = t; k < NE; k += kBlock) {
         short2 e = ledge[k];
-        const int ge = ledge_id[k];
-        double2 ds = d.eds[ge];
+        double2 ds = ledge_ds[k];
         double2 x1 = s_coord[e.x], x2 = s_coord[e.y];
         double4 a1 = s_prim[e.x], a2 = s_prim[e.y];
         double4 g1 = s_gx[e.x], h1 = s_gy[e.x], g2 = s_gx[e.y], h2 = s_gy[e.y];
@@ -673,7 +755,7 @@ __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, 
             else hllc_flux<1, float>(c, sxf, syf, pl, pr, ff);
             f = to_d(ff);
         }
-        if (!ok) bad = min(bad, ge + 1);
+        if (!ok) bad = min(bad, ledge_id[k] + 1);
         reinterpret_cast<double4 *>(s_work)[k] = make_double4(f.v[0], f.v[1], f.v[2], f.v[3]);
     }
     for (int k = t; k < NB; k += kBlock) {
@@ -702,13 +784,17 @@ __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, 
     Red red{0.0, 0.0, 0};
     if (t < NP) {
         const int v = v0 + t;
+        // loads of the update, issued before the ordered sum
+        double4 o = make_double4(0.0, 0.0, 0.0, 0.0);
+        double dtv = 0.0, cva = 1.0;
+        if (MODE != 2) { o = d.prim[v]; dtv = d.dt[v]; cva = d.cvarea[v]; }
         V4 r;
         r.v[0] = 0.0;
         r.v[1] = VISC ? rv1 : 0.0; r.v[2] = VISC ? rv2 : 0.0; r.v[3] = VISC ? rv3 : 0.0;
-        const int k1 = pptr[t + 1];
+        const int k1 = s_pptr[t + 1];
         const double4 *fl = reinterpret_cast<const double4 *>(s_work);
-        for (int k = pptr[t]; k < k1; ++k) {
-            const int q = pslot[k];
+        for (int k = s_pptr[t]; k < k1; ++k) {
+            const int q = s_psl[k];
             double4 f = fl[q >> 1];
             if (q & 1) {
                 r.v[0] = r.v[0] - f.x; r.v[1] = r.v[1] - f.y; r.v[2] = r.v[2] - f.z; r.v[3] = r.v[3] - f.w;
@@ -719,13 +805,12 @@ __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, 
         if (MODE == 2) {
             st4(res_out, v, r);
         } else {
-            const double4 o = d.prim[v];             // state at the start of the iteration
-            const double4 pc = s_prim[t];
+            const double4 pc = s_prim[t];            // o: state at the start of the iteration
             V4 po{{o.x, o.y, o.z, o.w}}, p{{pc.x, pc.y, pc.z, pc.w}}, u0, u1, un;
             prim2con(c, po, u0);
             prim2con(c, p, u1);
             const double a = c.airk[irk - 1], bb = c.birk[irk - 1];
-            const double f = d.dt[v] / d.cvarea[v];
+            const double f = dtv / cva;
 #pragma unroll
             for (int j = 0; j < 4; ++j) un.v[j] = a * u0.v[j] + bb * (u1.v[j] - f * r.v[j]);
             if (VISC && d.solid[v]) { un.v[1] = 0.0; un.v[2] = 0.0; }
@@ -760,13 +845,22 @@ __global__ void __launch_bounds__(kBlock) k_patch(Dev d, Consts c, DevState *S, 
         }
     }
     if (MODE == 1) {
-        Red rr = block_reduce(red);
+        Red rr = block_reduce<kBlock>(red);
         if (t == 0) {
             d.partial[3 * b] = rr.s;
             d.partial[3 * b + 1] = rr.m;
             d.partial[3 * b + 2] = (double)rr.i;
         }
     }
+}
+
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
+__global__ void __launch_bounds__(kBlock, VISC ? kPatchWavesVisc : kPatchWavesEuler)
+k_patch(Dev d, Consts c, DevState *S, PatchArgs pa, int irk, const double4 *__restrict__ pin,
+        double4 *__restrict__ pout, double4 *res_out)
+{
+    patch_body<FLUX, LIM, VISC, MODE, TURB, PREC>(d, c, S, pa, logical_block(pa.npatch, d.xcd), irk,
+                                                  pin, pout, res_out);
 }
 
 // Wall distance, src/wall_dist.f:28-61: for every point the smallest
@@ -822,11 +916,124 @@ __global__ void k_farfield_state(Dev d, Consts c, double cl)
     d.uout[b] = make_double4(u[0], u[1], u[2], u[3]);
 }
 
+// with the point vortex: far-field states of the next iteration from this
+// iteration's cl (COMMON cl, src/farfield_flux.f:28-44)
+__device__ __forceinline__ void vortex_body(const Dev &d, const Consts &c, const DevState *S, int b)
+{
+    if (S->err_code) return;
+    if (b >= d.nff) return;
+    int2 e = d.bnode[d.nsw + b];
+    double u[4];
+    farfield_state(c, S->cl, d.coord[e.x], d.coord[e.y], u);
+    d.uout[b] = make_double4(u[0], u[1], u[2], u[3]);
+}
+
+__global__ void k_vortex_state(Dev d, Consts c, const DevState *S)
+{
+    vortex_body(d, c, S, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ------------------------------------------------- batches of solves
+// A batch runs many solves of one kind (flow type, flux, limiter, precision)
+// with one launch per RK stage: the workgroups of all members' patches form
+// one work list of (member, patch) pairs, and each workgroup reads its
+// member's descriptor.  A single solve runs as a batch of one.  Members that
+// have stopped (converged, iteration limit, error) leave at once.
+struct BatchMember {
+    Dev d;
+    Consts c;
+    DevState *S;
+    PatchArgs pa;
+    double4 *pbuf[3];      // stage buffers O, B, C
+};
+
+// Descriptors and work lists are read through the constant address space:
+// they do not change during a launch, so the compiler treats them like
+// kernel arguments (scalar loads, reloaded where needed instead of held in
+// vector registers across the kernel's stores).
+template <class T>
+__device__ __forceinline__ const T &uniform_at(const T *p, int i)
+{
+    using CT = __attribute__((address_space(4))) const T;
+    return *(const T *)((const CT *)p + i);
+}
+
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
+__global__ void __launch_bounds__(kBlock, VISC ? kPatchWavesVisc : kPatchWavesEuler)
+k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work, int nwork, int xcd,
+              int irk)
+{
+    const int2 w = uniform_at(work, logical_block(nwork, xcd));
+    const BatchMember &M = uniform_at(mem, w.x);
+    patch_body<FLUX, LIM, VISC, MODE, TURB, PREC>(M.d, M.c, M.S, M.pa, w.y, irk, M.pbuf[irk - 1],
+                                                  M.pbuf[irk % 3], nullptr);
+}
+
+// vertex kernels over a work list of (member, block of kBlock vertices)
+template <bool VISC>
+__global__ void __launch_bounds__(kBlock) k_tstep_batch(const BatchMember *__restrict__ mem,
+                                                        const int2 *__restrict__ vwork)
+{
+    const int2 w = uniform_at(vwork, (int)blockIdx.x);
+    const BatchMember &M = uniform_at(mem, w.x);
+    tstep_body<VISC>(M.d, M.c, w.y * kBlock + threadIdx.x);
+}
+
+template <bool COMPAT>
+__global__ void __launch_bounds__(kBlock) k_sa_batch(const BatchMember *__restrict__ mem,
+                                                     const int2 *__restrict__ vwork)
+{
+    const int2 w = uniform_at(vwork, (int)blockIdx.x);
+    const BatchMember &M = uniform_at(mem, w.x);
+    sa_body<COMPAT>(M.d, M.c, M.S, w.y * kBlock + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy_batch(const BatchMember *__restrict__ mem,
+                                                       const int2 *__restrict__ vwork)
+{
+    const int2 w = uniform_at(vwork, (int)blockIdx.x);
+    const BatchMember &M = uniform_at(mem, w.x);
+    if (M.S->done) return;
+    const int v = w.y * kBlock + threadIdx.x;
+    if (v < M.d.np) M.d.nut[v] = M.d.nut_new[v];
+}
+
+// one wide workgroup per member
+template <bool VISC>
+__global__ void __launch_bounds__(kFinal) k_final_batch(const BatchMember *__restrict__ mem)
+{
+    const BatchMember &M = uniform_at(mem, (int)blockIdx.x);
+    final_body<VISC>(M.d, M.c, M.S);
+}
+
+__global__ void __launch_bounds__(kBlock) k_vortex_batch(const BatchMember *__restrict__ mem)
+{
+    const BatchMember &M = uniform_at(mem, (int)blockIdx.y);
+    if (M.c.vortex == 1 && M.c.mach_inf < 1.0)
+        vortex_body(M.d, M.c, M.S, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ void k_begin_batch(const BatchMember *__restrict__ mem, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) mem[i].S->nhist = 0;
+}
+
+__global__ void k_gather_batch(const BatchMember *__restrict__ mem, int n, DevState *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = *mem[i].S;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- host side
 
 struct fvm_ctx {
+    int cus = 256, per_cu = 4, lds_per_cu = 160 * 1024;   // device and occupancy
+    int target = 0;                                         // patch size asked of the partition
+    int max_patch = 0;
+    long long halo_total = 0;
     fvm_params par;
     Consts c;
     Dev d;
@@ -845,6 +1052,24 @@ struct fvm_ctx {
     double4 *pbuf[3] = {nullptr, nullptr, nullptr};   // stage buffers O, B, C
     PatchArgs pa{};
     std::vector<int> perm;            // device vertex -> caller's vertex (0-based)
+    BatchMember *self_mem = nullptr;  // this solve as a batch of one
+    int2 *self_work = nullptr;
+    struct fvm_batch *batch = nullptr;   // the batch it belongs to, if any
+};
+
+// Many solves of one kind sharing each launch (see BatchMember).  Membership
+// changes rebuild the work lists and the captured iterations.
+struct fvm_batch {
+    hipStream_t stream = nullptr;
+    std::vector<fvm_ctx *> mem;
+    BatchMember *d_mem = nullptr;
+    int2 *d_work = nullptr, *d_vwork = nullptr;
+    DevState *d_states = nullptr;
+    int nwork = 0, nvwork = 0, lds = 0, max_nff = 0;
+    bool dirty = true, any_vortex = false;
+    hipGraphExec_t graph_n = nullptr, graph_1 = nullptr;
+    bool graphs_ok = true;
+    std::vector<DevState> hs;
 };
 
 namespace {
@@ -910,6 +1135,8 @@ void set_consts(fvm_ctx *x)
     c.p_inf = 1.0 / (c.GAMMA * (p.mach_inf * p.mach_inf));
     c.T_inf = c.p_inf / (c.r_inf * 1.0);
     c.aoa = p.aoa_deg * c.M_PI_ / 180.0;
+    c.sin_aoa = std::sin(c.aoa);
+    c.cos_aoa = std::cos(c.aoa);
     c.u_inf = c.q_inf * std::cos(c.aoa);
     c.v_inf = c.q_inf * std::sin(c.aoa);
     c.a_inf = std::sqrt(c.GAMMA * c.p_inf / c.r_inf);
@@ -999,64 +1226,96 @@ Dev stage_dev(const fvm_ctx *x, const double4 *pin)
     return d;
 }
 
-template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
-void launch_patch_t(fvm_ctx *x, DevState *S, int irk, const double4 *pin, double4 *pout, double4 *res)
-{
-    if (x->par.precision == 1)
-        hipLaunchKernelGGL((k_patch<FLUX, LIM, VISC, MODE, TURB, 1>), dim3(x->pa.npatch), dim3(kBlock),
-                           x->pa.lds_bytes, x->stream, x->d, x->c, S, x->pa, irk, pin, pout, res);
-    else
-        hipLaunchKernelGGL((k_patch<FLUX, LIM, VISC, MODE, TURB, 0>), dim3(x->pa.npatch), dim3(kBlock),
-                           x->pa.lds_bytes, x->stream, x->d, x->c, S, x->pa, irk, pin, pout, res);
-}
+template <int V> using IC = std::integral_constant<int, V>;
 
-template <template <int, int, bool, int, bool> class L, int FLUX, int LIM>
-void dispatch_fl(fvm_ctx *x, DevState *S, int irk, int mode, const double4 *pin, double4 *pout,
-                 double4 *res)
+// Calls f(FLUX, LIM, VISC, TURB, PREC) with compile-time constants for one
+// kind of solve: flux 1 Roe, 2 KFVS, 3 HLLC, 4 HLLC rotated; limiter 0/1;
+// flow 1 Euler, 2 laminar, 3 SA; precision 0 FP64, 1 split.
+template <class F>
+void with_kind(int iflow, int iflux, int hllc_rotate, int ilimit, int precision, F &&f)
 {
-    const bool visc = x->par.iflow != 1, turb = x->par.iflow == 3;
-    if (!visc) {
-        if (mode == 0) L<FLUX, LIM, false, 0, false>::go(x, S, irk, pin, pout, res);
-        else if (mode == 1) L<FLUX, LIM, false, 1, false>::go(x, S, irk, pin, pout, res);
-        else L<FLUX, LIM, false, 2, false>::go(x, S, irk, pin, pout, res);
-    } else if (!turb) {
-        if (mode == 0) L<FLUX, LIM, true, 0, false>::go(x, S, irk, pin, pout, res);
-        else if (mode == 1) L<FLUX, LIM, true, 1, false>::go(x, S, irk, pin, pout, res);
-        else L<FLUX, LIM, true, 2, false>::go(x, S, irk, pin, pout, res);
-    } else {
-        if (mode == 0) L<FLUX, LIM, true, 0, true>::go(x, S, irk, pin, pout, res);
-        else if (mode == 1) L<FLUX, LIM, true, 1, true>::go(x, S, irk, pin, pout, res);
-        else L<FLUX, LIM, true, 2, true>::go(x, S, irk, pin, pout, res);
+    auto prec = [&](auto fl, auto lm, auto vi, auto tu) {
+        if (precision == 1) f(fl, lm, vi, tu, IC<1>{});
+        else f(fl, lm, vi, tu, IC<0>{});
+    };
+    auto flow = [&](auto fl, auto lm) {
+        if (iflow == 1) prec(fl, lm, IC<0>{}, IC<0>{});
+        else if (iflow == 2) prec(fl, lm, IC<1>{}, IC<0>{});
+        else prec(fl, lm, IC<1>{}, IC<1>{});
+    };
+    auto lim = [&](auto fl) {
+        if (ilimit == 1) flow(fl, IC<1>{});
+        else flow(fl, IC<0>{});
+    };
+    switch (iflux == 3 && hllc_rotate ? 4 : iflux) {
+    case 1: lim(IC<1>{}); break;
+    case 2: lim(IC<2>{}); break;
+    case 3: lim(IC<3>{}); break;
+    default: lim(IC<4>{}); break;
     }
 }
 
-template <template <int, int, bool, int, bool> class L>
-void dispatch(fvm_ctx *x, DevState *S, int irk, int mode, const double4 *pin, double4 *pout,
-              double4 *res)
-{
-    int f = x->par.iflux == 3 && x->par.hllc_rotate ? 4 : x->par.iflux;
-    switch (f * 2 + (x->par.ilimit == 1 ? 1 : 0)) {
-    case 2: dispatch_fl<L, 1, 0>(x, S, irk, mode, pin, pout, res); break;
-    case 3: dispatch_fl<L, 1, 1>(x, S, irk, mode, pin, pout, res); break;
-    case 4: dispatch_fl<L, 2, 0>(x, S, irk, mode, pin, pout, res); break;
-    case 5: dispatch_fl<L, 2, 1>(x, S, irk, mode, pin, pout, res); break;
-    case 6: dispatch_fl<L, 3, 0>(x, S, irk, mode, pin, pout, res); break;
-    case 7: dispatch_fl<L, 3, 1>(x, S, irk, mode, pin, pout, res); break;
-    case 8: dispatch_fl<L, 4, 0>(x, S, irk, mode, pin, pout, res); break;
-    default: dispatch_fl<L, 4, 1>(x, S, irk, mode, pin, pout, res); break;
-    }
-}
-
-template <int FLUX, int LIM, bool VISC, int MODE, bool TURB>
-struct PatchLaunch {
-    static void go(fvm_ctx *x, DevState *S, int irk, const double4 *pin, double4 *pout, double4 *res)
-    { launch_patch_t<FLUX, LIM, VISC, MODE, TURB>(x, S, irk, pin, pout, res); }
+// what one launch of the batched stage kernel needs
+struct StageLaunch {
+    const BatchMember *mem;
+    const int2 *work;
+    int nwork, xcd, lds;
+    hipStream_t stream;
+    int iflow, iflux, hllc_rotate, ilimit, precision;
 };
 
-void launch_patch(fvm_ctx *x, DevState *S, int irk, int mode, const double4 *pin, double4 *pout,
-                  double4 *res)
+void launch_stage(const StageLaunch &b, int irk, int mode)
 {
-    dispatch<PatchLaunch>(x, S, irk, mode, pin, pout, res);
+    with_kind(b.iflow, b.iflux, b.hllc_rotate, b.ilimit, b.precision,
+              [&](auto fl, auto lm, auto vi, auto tu, auto pr) {
+        constexpr int F = decltype(fl)::value, L = decltype(lm)::value, P = decltype(pr)::value;
+        constexpr bool V = decltype(vi)::value != 0, T = decltype(tu)::value != 0;
+        if (mode == 0)
+            hipLaunchKernelGGL((k_patch_batch<F, L, V, 0, T, P>), dim3(b.nwork), dim3(kBlock), b.lds,
+                               b.stream, b.mem, b.work, b.nwork, b.xcd, irk);
+        else
+            hipLaunchKernelGGL((k_patch_batch<F, L, V, 1, T, P>), dim3(b.nwork), dim3(kBlock), b.lds,
+                               b.stream, b.mem, b.work, b.nwork, b.xcd, irk);
+    });
+}
+
+// residual of one state through the stage kernel (MODE 2, no update)
+void launch_residual(fvm_ctx *x, DevState *S, const double4 *pin, double4 *res)
+{
+    const fvm_params &q = x->par;
+    with_kind(q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision,
+              [&](auto fl, auto lm, auto vi, auto tu, auto pr) {
+        constexpr int F = decltype(fl)::value, L = decltype(lm)::value, P = decltype(pr)::value;
+        constexpr bool V = decltype(vi)::value != 0, T = decltype(tu)::value != 0;
+        hipLaunchKernelGGL((k_patch<F, L, V, 2, T, P>), dim3(x->pa.npatch), dim3(kBlock),
+                           x->pa.lds_bytes, x->stream, x->d, x->c, S, x->pa, 1, pin, x->pbuf[1], res);
+    });
+}
+
+StageLaunch self_stage(const fvm_ctx *x)
+{
+    const fvm_params &q = x->par;
+    return StageLaunch{x->self_mem, x->self_work, x->pa.npatch, x->d.xcd, x->pa.lds_bytes, x->stream,
+                       q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision};
+}
+
+BatchMember member_of(const fvm_ctx *x)
+{
+    BatchMember m;
+    m.d = x->d;
+    m.c = x->c;
+    m.S = x->S;
+    m.pa = x->pa;
+    for (int i = 0; i < 3; ++i) m.pbuf[i] = x->pbuf[i];
+    return m;
+}
+
+// the context's own descriptor, after anything in it changed
+int sync_self(fvm_ctx *x)
+{
+    BatchMember m = member_of(x);
+    HIPCHK(hipMemcpy(x->self_mem, &m, sizeof m, hipMemcpyHostToDevice));
+    return FVM_OK;
 }
 
 void launch_tstep(fvm_ctx *x)
@@ -1098,7 +1357,8 @@ void one_iteration(fvm_ctx *x, Tally *tally)
             const double4 *pin = x->pbuf[irk - 1];
             double4 *pout = x->pbuf[irk % 3];
             const int mode = irk == 3 ? 1 : 0;
-            timed("k_patch", [&] { launch_patch(x, x->S, irk, mode, pin, pout, nullptr); });
+            (void)pin; (void)pout;
+            timed("k_patch", [&] { launch_stage(self_stage(x), irk, mode); });
         }
     } else {
     timed("k_tstep", [&] { launch_tstep(x); });
@@ -1132,7 +1392,13 @@ void one_iteration(fvm_ctx *x, Tally *tally)
         });
     }
     timed("k_final", [&] {
-        hipLaunchKernelGGL(k_final, dim3(1), dim3(kBlock), 0, x->stream, x->d, x->c, x->S);
+        if (x->par.iflow == 1)
+            hipLaunchKernelGGL(k_final<false>, dim3(1), dim3(kFinal), 0, x->stream, x->d, x->c, x->S);
+        else
+            hipLaunchKernelGGL(k_final<true>, dim3(1), dim3(kFinal), 0, x->stream, x->d, x->c, x->S);
+        if (x->c.vortex == 1 && x->c.mach_inf < 1.0 && x->nff > 0)
+            hipLaunchKernelGGL(k_vortex_state, dim3(nblk(x->nff)), dim3(kBlock), 0, x->stream, x->d,
+                               x->c, x->S);
     });
     if (tally) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); }
 }
@@ -1150,6 +1416,11 @@ int configure(fvm_ctx *x, int variant, int vb, int xcd)
     x->d.nblocks = (x->np + vb - 1) / vb;
     x->d.xcd = xcd ? 1 : 0;
     x->d.nparts = variant == 3 ? x->pa.npatch : (int)nblk(x->np);
+    if (x->self_mem) {
+        int rc = sync_self(x);
+        if (rc) return rc;
+    }
+    if (x->batch) x->batch->dirty = true;
     if (x->graph_n) (void)hipGraphExecDestroy(x->graph_n);
     if (x->graph_1) (void)hipGraphExecDestroy(x->graph_1);
     x->graph_n = x->graph_1 = nullptr;
@@ -1273,6 +1544,8 @@ struct PatchHost {
     std::vector<short4> ltri;
     std::vector<short> gtl, pslot;
     std::vector<short2> ledge, lbnd;
+    std::vector<double2> ledge_ds;
+    std::vector<double> larea;
     int lds_max = 0;
 };
 
@@ -1381,9 +1654,7 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
         if (NL > 32767 || 4 * (int)tl.size() > 32767 || 2 * (NE + 2 * NB) > 32767) return FVM_E_SIZE;
         h[PH_NGTL] = (int)ph.gtl.size() - h[PH_GTL];
         h[PH_NPSL] = (int)ph.pslot.size() - h[PH_PSL];
-        const int NLa = (NL + 1) & ~1;
-        const int lds = NLa * 48 + (visc ? NLa * 24 : 0) + NG * 64 +
-                        std::max((int)tl.size() * (visc ? 14 : 8) * 8, (NE + 2 * NB) * 32);
+        const int lds = patch_lds_bytes(NP, NL, NG, (int)tl.size(), NE, NB, h[PH_NGTL], h[PH_NPSL], visc);
         ph.lds_max = std::max(ph.lds_max, lds);
         for (int i = 0; i < kPatchHdr; ++i) ph.hdr.push_back(h[i]);
         for (int v : lv) loc[v] = -1;
@@ -1408,6 +1679,7 @@ struct Layout {
     std::vector<double> ais, cvarea, drmin;
     std::vector<unsigned char> solid;
     PatchHost ph;
+    int target = 0;                                // patch size asked of the partition
 };
 
 int build_layout(const fvm_mesh *m, int iflow, int patch_size, int lds_limit, Layout &L)
@@ -1431,6 +1703,7 @@ int build_layout(const fvm_mesh *m, int iflow, int patch_size, int lds_limit, La
         }
     }
     fvm::Partition part = fvm::partition_vertices(np, adj_ptr, adj, target);
+    L.target = target;
     std::vector<int> &perm = part.perm;            // new -> old
     std::vector<int> iperm(np);                    // old -> new
     for (int n = 0; n < np; ++n) iperm[perm[n]] = n;
@@ -1552,6 +1825,7 @@ int validate_layout(const fvm_mesh *m, const Layout &L)
         const int *h = &ph.hdr[kPatchHdr * k];
         const int v0 = h[PH_V0], NP = h[PH_NP], NL = h[PH_NL], NG = h[PH_NG];
         const int NT = h[PH_NTRI], NE = h[PH_NEDGE], NB = h[PH_NBND];
+    const int NGTL = h[PH_NGTL], NPSL = h[PH_NPSL];
         if (h[PH_NGTL] != ph.gptr[h[PH_GPTR] + NG] || h[PH_NPSL] != ph.pptr[h[PH_PPTR] + NP]) return 25;
         if (v0 != L.patch_ptr[k] || NP != L.patch_ptr[k + 1] - v0 || NP < 1 || NP > kBlock) return 8;
         if (NG < NP || NL < NG || h[PH_HALO] + (NL - NP) > (int)ph.halo.size()) return 9;
@@ -1679,6 +1953,20 @@ int fvm_patch_plan(const fvm_mesh *m, int iflow, int patch_size, int lds_limit, 
     return st[7] ? FVM_E_INPUT : FVM_OK;
 }
 
+int fvm_gpu_layout_info(const fvm_ctx *x, long long *info)
+{
+    const int lds = x->pa.lds_bytes + 1024;
+    info[0] = x->pa.npatch;
+    info[1] = x->max_patch;
+    info[2] = x->pa.lds_bytes;
+    info[3] = std::min(x->per_cu, x->lds_per_cu / lds);
+    info[4] = x->cus;
+    info[5] = x->target;
+    info[6] = x->halo_total;
+    info[7] = x->variant;
+    return FVM_OK;
+}
+
 long long fvm_gpu_footprint(const fvm_ctx *x) { return x ? x->bytes : 0; }
 
 int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
@@ -1699,12 +1987,33 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     set_consts(x);
     if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) { delete x; return FVM_E_HIP; }
 
-    // LDS a workgroup may use on this device, less the static LDS
-    int dev = 0, lds_dev = 64 * 1024;
-    if (hipGetDevice(&dev) == hipSuccess)
+    // LDS a workgroup may use on this device (less the static LDS), CUs and
+    // LDS per CU for the automatic patch size
+    int dev = 0, lds_dev = 64 * 1024, cus = 256, lds_cu = 160 * 1024;
+    if (hipGetDevice(&dev) == hipSuccess) {
         (void)hipDeviceGetAttribute(&lds_dev, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+    }
+    x->cus = cus;
+    x->per_cu = par->iflow == 1 ? kPatchWavesEuler : kPatchWavesVisc;
     Layout L;
-    rc = build_layout(m, par->iflow, par->patch_size, lds_dev - 6 * 1024, L);
+    if (par->patch_size > 0) {
+        rc = build_layout(m, par->iflow, par->patch_size, lds_dev - 6 * 1024, L);
+    } else {
+        // Whole rounds of resident workgroups, about 64 vertices a patch:
+        // k rounds of cus * per_cu workgroups; fewer vertices per patch
+        // while the LDS would keep fewer than per_cu workgroups on a CU.
+        const long slots = (long)cus * x->per_cu;
+        long k = std::max(1L, std::lround(np / (64.0 * slots)));
+        for (int tries = 0;; ++tries, ++k) {
+            const int ps = (int)std::min(128L, std::max(24L, (np + k * slots - 1) / (k * slots)));
+            L = Layout();
+            rc = build_layout(m, par->iflow, ps, lds_dev - 6 * 1024, L);
+            if (rc || tries == 3 || ps == 24 || (L.ph.lds_max + 1024) * x->per_cu <= lds_cu) break;
+        }
+    }
+    if (!rc) x->lds_per_cu = lds_cu;
     if (!rc && validate_layout(m, L)) rc = FVM_E_INPUT;
     if (rc) { fvm_gpu_destroy(x); return rc; }
     x->perm = L.perm;
@@ -1736,6 +2045,14 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     TRY(upload(x, (int **)&x->pa.halo, L.ph.halo));
     TRY(upload(x, (short4 **)&x->pa.ltri, L.ph.ltri));
     TRY(upload(x, (int **)&x->pa.gtri, L.ph.gtri));
+    {
+        std::vector<double> larea(L.ph.gtri.size());
+        for (size_t i = 0; i < larea.size(); ++i) larea[i] = m->elarea[L.ph.gtri[i]];
+        TRY(upload(x, (double **)&x->pa.larea, larea));
+        std::vector<double2> lds(L.ph.ledge_id.size());
+        for (size_t i = 0; i < lds.size(); ++i) lds[i] = L.eds[L.ph.ledge_id[i]];
+        TRY(upload(x, (double2 **)&x->pa.ledge_ds, lds));
+    }
     TRY(upload(x, (int **)&x->pa.gptr, L.ph.gptr));
     TRY(upload(x, (short **)&x->pa.gtl, L.ph.gtl));
     TRY(upload(x, (short2 **)&x->pa.ledge, L.ph.ledge));
@@ -1746,6 +2063,15 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     TRY(upload(x, (short **)&x->pa.pslot, L.ph.pslot));
     x->pa.npatch = (int)L.patch_ptr.size() - 1;
     x->pa.lds_bytes = L.ph.lds_max;
+    x->target = L.target;
+    {
+        long long halo = 0; int maxp = 0;
+        for (int k = 0; k < x->pa.npatch; ++k) {
+            halo += L.ph.hdr[kPatchHdr * k + PH_NL] - L.ph.hdr[kPatchHdr * k + PH_NP];
+            maxp = std::max(maxp, L.ph.hdr[kPatchHdr * k + PH_NP]);
+        }
+        x->halo_total = halo; x->max_patch = maxp;
+    }
     TRY(dalloc(x, &d.prim, np));
     TRY(dalloc(x, &x->pbuf[1], np));
     TRY(dalloc(x, &x->pbuf[2], np));
@@ -1778,6 +2104,12 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
         TRY(upload(x, (double **)&d.wd, wd));
         TRY(dalloc(x, &d.nut_new, np));
     }
+    TRY(dalloc(x, &x->self_mem, 1));
+    {
+        std::vector<int2> w(x->pa.npatch);
+        for (int k = 0; k < x->pa.npatch; ++k) w[k] = make_int2(0, k);
+        TRY(upload(x, (int2 **)&x->self_work, w));
+    }
 #undef TRY
 
     // ---- free-stream start (src/initialize.f:60-90) ----
@@ -1806,6 +2138,7 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
 int fvm_gpu_destroy(fvm_ctx *x)
 {
     if (!x) return FVM_OK;
+    if (x->batch) (void)fvm_batch_remove(x->batch, x);
     if (x->stream) (void)hipStreamSynchronize(x->stream);
     drop_graphs(x);
     for (void *p : x->allocs) (void)hipFree(p);
@@ -1931,7 +2264,7 @@ int fvm_gpu_residu(fvm_ctx *x, const double *prim, const double *nut, double *re
     h.cl = x->hS.cl;
     HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
     if (x->variant == 3) {
-        launch_patch(x, x->S_hook, 1, 2, x->d.prim, x->pbuf[1], x->res_tmp);
+        launch_residual(x, x->S_hook, x->d.prim, x->res_tmp);
     } else {
         residual_front(x, x->S_hook);
         if (x->par.iflow != 1)
@@ -2008,3 +2341,210 @@ int fvm_gpu_time_kernels(fvm_ctx *x, int n, double *ms, const char **names, int 
 }
 
 }  // extern "C"
+
+
+// ----------------------------------------------------------------- batches
+
+namespace {
+
+void batch_free(fvm_batch *b)
+{
+    if (b->graph_n) (void)hipGraphExecDestroy(b->graph_n);
+    if (b->graph_1) (void)hipGraphExecDestroy(b->graph_1);
+    b->graph_n = b->graph_1 = nullptr;
+    for (void *p : {(void *)b->d_mem, (void *)b->d_work, (void *)b->d_vwork, (void *)b->d_states})
+        if (p) (void)hipFree(p);
+    b->d_mem = nullptr; b->d_work = b->d_vwork = nullptr; b->d_states = nullptr;
+    b->nwork = b->nvwork = 0;
+}
+
+template <class T>
+int batch_upload(T **p, const std::vector<T> &h)
+{
+    if (hipMalloc((void **)p, std::max<size_t>(h.size(), 1) * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return FVM_E_NOMEM;
+    }
+    if (!h.empty()) HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return FVM_OK;
+}
+
+// descriptors and work lists of the current members
+int batch_rebuild(fvm_batch *b)
+{
+    HIPCHK(hipStreamSynchronize(b->stream));
+    batch_free(b);
+    std::vector<BatchMember> m;
+    std::vector<int2> work, vwork;
+    b->lds = 0; b->max_nff = 0; b->any_vortex = false;
+    for (int i = 0; i < (int)b->mem.size(); ++i) {
+        const fvm_ctx *x = b->mem[i];
+        m.push_back(member_of(x));
+        for (int k = 0; k < x->pa.npatch; ++k) work.push_back(make_int2(i, k));
+        for (int k = 0; k < (int)nblk(x->np); ++k) vwork.push_back(make_int2(i, k));
+        b->lds = std::max(b->lds, x->pa.lds_bytes);
+        b->max_nff = std::max(b->max_nff, x->nff);
+        if (x->c.vortex == 1 && x->c.mach_inf < 1.0) b->any_vortex = true;
+    }
+    int rc = batch_upload(&b->d_mem, m);
+    if (!rc) rc = batch_upload(&b->d_work, work);
+    if (!rc) rc = batch_upload(&b->d_vwork, vwork);
+    if (!rc && hipMalloc((void **)&b->d_states, std::max<size_t>(m.size(), 1) * sizeof(DevState)) != hipSuccess)
+        rc = FVM_E_NOMEM;
+    if (rc) { batch_free(b); return rc; }
+    b->nwork = (int)work.size();
+    b->nvwork = (int)vwork.size();
+    b->hs.assign(m.size(), DevState{});
+    b->graphs_ok = true;
+    b->dirty = false;
+    return FVM_OK;
+}
+
+void batch_iteration(fvm_batch *b)
+{
+    const fvm_params &q = b->mem[0]->par;
+    const int nm = (int)b->mem.size();
+    const bool turb = q.iflow == 3;
+    if (turb)
+        hipLaunchKernelGGL(k_tstep_batch<true>, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
+    StageLaunch L{b->d_mem, b->d_work, b->nwork, 1, b->lds, b->stream,
+                  q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision};
+    for (int irk = 1; irk <= 3; ++irk) launch_stage(L, irk, irk == 3 ? 1 : 0);
+    if (turb) {
+        if (q.sa_compat)
+            hipLaunchKernelGGL(k_sa_batch<true>, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
+        else
+            hipLaunchKernelGGL(k_sa_batch<false>, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
+        hipLaunchKernelGGL(k_copy_batch, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
+    }
+    if (q.iflow == 1)
+        hipLaunchKernelGGL(k_final_batch<false>, dim3(nm), dim3(kFinal), 0, b->stream, b->d_mem);
+    else
+        hipLaunchKernelGGL(k_final_batch<true>, dim3(nm), dim3(kFinal), 0, b->stream, b->d_mem);
+    if (b->any_vortex && b->max_nff > 0)
+        hipLaunchKernelGGL(k_vortex_batch, dim3(nblk(b->max_nff), nm), dim3(kBlock), 0, b->stream, b->d_mem);
+}
+
+int batch_graph(fvm_batch *b, int iters, hipGraphExec_t *out)
+{
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return FVM_E_HIP;
+    for (int i = 0; i < iters; ++i) batch_iteration(b);
+    if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return FVM_E_HIP;
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    return e == hipSuccess ? FVM_OK : FVM_E_HIP;
+}
+
+bool same_kind(const fvm_params &a, const fvm_params &b)
+{
+    return a.iflow == b.iflow && a.iflux == b.iflux && a.ilimit == b.ilimit && a.precision == b.precision &&
+           a.hllc_rotate == b.hllc_rotate && a.sa_compat == b.sa_compat;
+}
+
+int member_status(const DevState &h)
+{
+    if (h.err_code) return h.err_code;
+    return h.done ? h.status : FVM_OK;
+}
+
+}  // namespace
+
+int fvm_batch_create(fvm_batch **out)
+{
+    *out = nullptr;
+    fvm_batch *b = new fvm_batch();
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return FVM_E_HIP; }
+    *out = b;
+    return FVM_OK;
+}
+
+int fvm_batch_add(fvm_batch *b, fvm_ctx *x)
+{
+    if (!b || !x || x->batch || x->variant != 3) return FVM_E_INPUT;
+    if (!b->mem.empty() && !same_kind(b->mem[0]->par, x->par)) return FVM_E_INPUT;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    b->mem.push_back(x);
+    x->batch = b;
+    b->dirty = true;
+    return FVM_OK;
+}
+
+int fvm_batch_remove(fvm_batch *b, fvm_ctx *x)
+{
+    if (!b || !x || x->batch != b) return FVM_E_INPUT;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->mem.erase(std::find(b->mem.begin(), b->mem.end(), x));
+    x->batch = nullptr;
+    b->dirty = true;
+    return FVM_OK;
+}
+
+int fvm_batch_size(const fvm_batch *b) { return b ? (int)b->mem.size() : 0; }
+
+int fvm_batch_iterate(fvm_batch *b, int n, int *status)
+{
+    const int nm = (int)b->mem.size();
+    if (nm == 0 || n <= 0) return nm == 0 && n > 0 ? FVM_E_INPUT : FVM_OK;
+    if (b->dirty) {
+        int rc = batch_rebuild(b);
+        if (rc) return rc;
+    }
+    int left = n;
+    bool running = false;
+    for (const fvm_ctx *x : b->mem) running = running || !x->hS.done;
+    while (left > 0 && running) {
+        const int chunk = std::min(left, kHistCap);
+        hipLaunchKernelGGL(k_begin_batch, dim3(nblk(nm)), dim3(kBlock), 0, b->stream, b->d_mem, nm);
+        if (b->graphs_ok && !b->graph_n) {
+            if (batch_graph(b, kGraphIters, &b->graph_n) || batch_graph(b, 1, &b->graph_1)) {
+                b->graphs_ok = false;
+                (void)hipGetLastError();
+            }
+        }
+        if (b->graphs_ok) {
+            const int q = chunk / kGraphIters, r = chunk % kGraphIters;
+            for (int i = 0; i < q; ++i) HIPCHK(hipGraphLaunch(b->graph_n, b->stream));
+            for (int i = 0; i < r; ++i) HIPCHK(hipGraphLaunch(b->graph_1, b->stream));
+        } else {
+            for (int i = 0; i < chunk; ++i) batch_iteration(b);
+        }
+        hipLaunchKernelGGL(k_gather_batch, dim3(nblk(nm)), dim3(kBlock), 0, b->stream, b->d_mem, nm, b->d_states);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(b->hs.data(), b->d_states, sizeof(DevState) * nm, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        running = false;
+        for (int i = 0; i < nm; ++i) {
+            b->mem[i]->hS = b->hs[i];
+            running = running || !b->hs[i].done;
+        }
+        left -= chunk;
+    }
+    if (status)
+        for (int i = 0; i < nm; ++i) status[i] = member_status(b->mem[i]->hS);
+    return FVM_OK;
+}
+
+int fvm_batch_destroy(fvm_batch *b)
+{
+    if (!b) return FVM_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (fvm_ctx *x : b->mem) x->batch = nullptr;
+    b->mem.clear();
+    batch_free(b);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return FVM_OK;
+}
+
+int fvm_gpu_monitor(const fvm_ctx *x, double *mon)
+{
+    const DevState &h = x->hS;
+    mon[0] = (double)h.iter;
+    mon[1] = h.fres;
+    mon[2] = (double)h.iresi;
+    mon[3] = h.fresi;
+    mon[4] = h.cl;
+    mon[5] = h.cd;
+    return member_status(h);
+}

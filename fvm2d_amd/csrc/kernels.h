@@ -17,6 +17,7 @@ struct Consts {
     double Cb1, Cw1, Cw2, Cv2, Cv11, Cw31, Cw32, kolm2, Cb2Sig1, Cb2Sig2;
     double q_inf, r_inf, p_inf, T_inf, aoa, u_inf, v_inf, a_inf, SCONST, nut_inf;
     double rey, cfl, mach_inf, xref, yref, minres;
+    double sin_aoa, cos_aoa;              // host libm, as the reference's dsin/dcos
     double airk[3], birk[3];
     int iflow, iflux, ilimit, vortex, sa_compat, hllc_rotate;
     int iterlast, maxiter, saveinterval, np;

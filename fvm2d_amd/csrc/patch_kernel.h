@@ -3,10 +3,11 @@
 //
 // A workgroup owns a patch P of consecutive vertices.  It loads the stage
 // state of P, its 1-ring H1 and its 2-ring H2 into LDS, then
-//   1. evaluates every triangle touching G = P + H1 once (Green-Gauss terms
-//      and, for viscous flow, the Galerkin stresses);
+//   1. viscous flow: evaluates every triangle touching G = P + H1 once
+//      (Green-Gauss terms and the Galerkin stresses);
 //   2. sums, for each vertex of G, its triangles in ascending triangle
-//      order -> gradients of G (src/green_gauss.f);
+//      order -> gradients of G (src/green_gauss.f); inviscid flow evaluates
+//      each triangle's Green-Gauss term in this sum, from LDS;
 //   3. evaluates every edge touching P once (MUSCL + Riemann flux) and every
 //      solid/farfield edge touching P (src/residu.f:37-88);
 //   4. sums, for each vertex of P, its edge and boundary terms in the
@@ -30,11 +31,13 @@ struct PatchArgs {
     const int *hdr;         // kPatchHdr ints per patch
     const int *halo;        // new vertex ids of H1 then H2
     const short4 *ltri;     // local nodes of each local triangle
-    const int *gtri;        // its triangle number (for elarea)
+    const int *gtri;        // its triangle number
+    const double *larea;    // its area (elarea)
     const int *gptr;        // per G-vertex: range in gtl (patch-relative)
     const short *gtl;       // 4 * local triangle + position of the vertex in it
     const short2 *ledge;    // local end points of each local edge
     const int *ledge_id;    // its position in the live-edge loop
+    const double2 *ledge_ds;  // its dual-face normal (ds)
     const short2 *lbnd;     // local end points of each local boundary edge
     const int *lbnd_id;     // its index among the boundary edges (solid first)
     const int *pptr;        // per P-vertex: range in pslot (patch-relative)
@@ -42,5 +45,16 @@ struct PatchArgs {
     int npatch;
     int lds_bytes;
 };
+
+// bytes of the LDS carve-up of one patch; the kernel and the planner both use it
+__host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, int NE, int NB,
+                                               int NGTL, int NPSL, bool visc)
+{
+    const int NLa = (NL + 1) & ~1;
+    const int work = (NT * (visc ? 14 : 1) * 8) > ((NE + 2 * NB) * 32) ? NT * (visc ? 14 : 1) * 8
+                                                                       : (NE + 2 * NB) * 32;
+    const int index = 4 * (NG + 1) + 4 * (NP + 1) + 2 * (NGTL + NPSL);
+    return NLa * 48 + (visc ? NLa * 24 : 0) + NG * 64 + work + ((index + 15) & ~15);
+}
 
 }  // namespace fvm

@@ -86,6 +86,20 @@ def lib():
     L.fvm_gpu_time_kernels.argtypes = [vp, ci, vp, ctypes.POINTER(ctypes.c_char_p), ci]
     L.fvm_patch_plan.restype = ci
     L.fvm_patch_plan.argtypes = [ctypes.POINTER(Mesh), ci, ci, ci, vp]
+    L.fvm_gpu_layout_info.argtypes = [vp, vp]
+    L.fvm_gpu_monitor.restype = ci
+    L.fvm_gpu_monitor.argtypes = [vp, vp]
+    L.fvm_batch_create.restype = ci
+    L.fvm_batch_create.argtypes = [ctypes.POINTER(vp)]
+    for f in ("fvm_batch_add", "fvm_batch_remove"):
+        getattr(L, f).restype = ci
+        getattr(L, f).argtypes = [vp, vp]
+    L.fvm_batch_size.restype = ci
+    L.fvm_batch_size.argtypes = [vp]
+    L.fvm_batch_iterate.restype = ci
+    L.fvm_batch_iterate.argtypes = [vp, ci, vp]
+    L.fvm_batch_destroy.restype = ci
+    L.fvm_batch_destroy.argtypes = [vp]
     L.fvm_gpu_footprint.restype = ctypes.c_longlong
     L.fvm_gpu_footprint.argtypes = [vp]
     L.fvm_gpu_footprint_estimate.restype = ctypes.c_longlong

@@ -45,7 +45,7 @@ class Inputs:
     precision: int = 0        # 0 FP64, 1 split FP32 flux / FP64 accumulation
     nut_factor: float = 0.1
     istart: int = 1
-    patch_size: int = 0       # vertices per patch of the stage kernel (0 = 128)
+    patch_size: int = 0       # vertices per patch of the stage kernel (0 = automatic)
 
     def params(self) -> Params:
         return Params(iflow=self.iflow, iflux=self.iflux, ilimit=self.ilimit, vortex=self.vortex,
@@ -278,3 +278,70 @@ class Solver:
 
     def footprint(self) -> int:
         return int(self.L.fvm_gpu_footprint(self.h))
+
+    def monitor(self) -> tuple[dict, int]:
+        """Monitors of the last iteration and the solve's status."""
+        m = np.zeros(6)
+        st = self.L.fvm_gpu_monitor(self.h, _p(m))
+        return dict(iter=int(m[0]), fres=m[1], iresi=int(m[2]), fresi=m[3], cl=m[4], cd=m[5]), int(st)
+
+    def layout_info(self) -> dict:
+        """Launch layout of the stage kernel (patches, LDS, occupancy)."""
+        a = np.zeros(8, dtype=np.int64)
+        self.L.fvm_gpu_layout_info(self.h, _p(a))
+        keys = ("patches", "max_patch", "lds_bytes", "workgroups_per_cu", "cus", "patch_size",
+                "halo_total", "variant")
+        return {k: int(v) for k, v in zip(keys, a)}
+
+
+class Batch:
+    """Solves of one kind advanced together, one launch per RK stage over all
+    their patches (fvm_batch_*).  Members compute bit for bit what they
+    would compute alone."""
+
+    def __init__(self):
+        self.L = gpu.lib()
+        h = ctypes.c_void_p()
+        rc = self.L.fvm_batch_create(ctypes.byref(h))
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_batch_create")
+        self.h = h
+        self.members: list[Solver] = []
+
+    def add(self, s: Solver):
+        rc = self.L.fvm_batch_add(self.h, s.h)
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_batch_add")
+        self.members.append(s)
+
+    def remove(self, s: Solver):
+        rc = self.L.fvm_batch_remove(self.h, s.h)
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_batch_remove")
+        self.members.remove(s)
+
+    def iterate(self, n: int) -> list[int]:
+        """Up to n iterations of every running member; statuses in member order."""
+        st = np.zeros(max(len(self.members), 1), dtype=np.int32)
+        rc = self.L.fvm_batch_iterate(self.h, n, _p(st))
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_batch_iterate")
+        return [int(v) for v in st[: len(self.members)]]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fvm_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -57,7 +57,8 @@ typedef struct {
     int precision;      /* 0 FP64 throughout; 1 split: FP32 reconstruction and
                            flux, FP64 geometry, accumulation, update, SA, norms */
     int patch_size;     /* vertices per patch of the LDS-tiled stage kernel
-                           (<= 256; 0 = default 128)                            */
+                           (<= 256; 0 = automatic: whole rounds of resident
+                           workgroups, about 64 vertices a patch)               */
     double nut_factor;  /* initial nu_tilde = nut_factor*mu_l/rho
                            (src/initialize.f:190 uses 0.1)                       */
 } fvm_params;
@@ -161,6 +162,38 @@ int fvm_patch_plan(const fvm_mesh *mesh, int iflow, int patch_size, int lds_limi
                    long long *stats);
 
 /* Bytes of device memory the context holds (admission by HBM footprint). */
+/* ---------------------------------------------------------------- batches
+ * No counterpart in the reference (it runs one solve per process).  A batch
+ * advances many contexts of one kind (same iflow, iflux, ilimit, precision,
+ * hllc_rotate, sa_compat; meshes, Mach numbers, angles and CFL may differ)
+ * with one launch per RK stage over all their patches, iterations captured
+ * in HIP graphs.  Each member computes exactly what fvm_gpu_iterate would:
+ * the same operations in the same order, bit for bit.  A context belongs to
+ * at most one batch; destroying it removes it. */
+typedef struct fvm_batch fvm_batch;
+
+int fvm_batch_create(fvm_batch **batch);
+int fvm_batch_add(fvm_batch *batch, fvm_ctx *ctx);     /* FVM_E_INPUT: other kind */
+int fvm_batch_remove(fvm_batch *batch, fvm_ctx *ctx);
+int fvm_batch_size(const fvm_batch *batch);
+/* Up to n iterations of every member that has not stopped (members stop on
+ * their own: MinRes, MaxIter, errors).  status[i] (member i in the order
+ * added, after removals): FVM_OK while running, FVM_CONVERGED,
+ * FVM_ITER_LIMIT or an error code. */
+int fvm_batch_iterate(fvm_batch *batch, int n, int *status);
+int fvm_batch_destroy(fvm_batch *batch);
+
+/* Monitors of the last iteration (src/main.f:63 RES.DAT row): mon[6] =
+ * iter, fres, iresi, fresi, cl, cd.  Returns the context's status as above. */
+int fvm_gpu_monitor(const fvm_ctx *ctx, double *mon);
+
+/* Device layout of a context: info[8] = patches, max patch vertices, LDS
+ * bytes per workgroup, resident workgroups per CU, CUs, patch size asked of
+ * the partition (patch_size, or the automatic choice when it is 0), halo
+ * vertices over all patches, kernel variant.  No counterpart in the
+ * reference (its loops have no launch shape); for tuning and reports. */
+int fvm_gpu_layout_info(const fvm_ctx *ctx, long long *info);
+
 long long fvm_gpu_footprint(const fvm_ctx *ctx);
 /* Predicted footprint before creating a context. */
 long long fvm_gpu_footprint_estimate(int np, int nt, int netot, int iflow, int precision);
