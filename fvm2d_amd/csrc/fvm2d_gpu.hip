@@ -1069,6 +1069,7 @@ struct fvm_batch {
     bool dirty = true, any_vortex = false;
     hipGraphExec_t graph_n = nullptr, graph_1 = nullptr;
     bool graphs_ok = true;
+    bool pending = false;             // a launched chunk not waited for yet
     std::vector<DevState> hs;
 };
 
@@ -1917,16 +1918,21 @@ const char *fvm_status_string(int s)
 
 long long fvm_gpu_footprint_estimate(int np, int nt, int netot, int iflow, int precision)
 {
-    (void)precision;
-    long long P = np, T = nt, E = netot;
+    (void)precision;                       // the same buffers in both precisions
+    const long long P = np, T = nt, E = netot;
     long long b = 0;
-    b += P * (16 + 32 * 4 + 8 * 4);        // coord, prim, prim_old, qx, qy, dt, cvarea, ais, drmin
-    b += P * 4 + 3 * T * 4 + T * 16;        // v2t_ptr, v2t, tri
-    b += E * (8 + 16 + 32) + P * 4 + 2 * E * 4;   // enode, eds, eflux, v2e
-    b += P * 5 + 4096;
-    if (iflow != 1) b += P * (32 + 8 * 3) + T * 8;
-    if (iflow == 3) b += P * 16;
-    b += 6LL * 8 * kHistCap + 3LL * 8 * ((np + kBlock - 1) / kBlock);
+    // per vertex: coord, stage buffers O/B/C, prim_old, qx, qy, residual and
+    // hook backups, dt, cvarea, ais, drmin, mu, mul, nut, nut_new, wd,
+    // solid, orig, adjacency offsets
+    b += P * (16 + 3 * 32 + 32 + 2 * 32 + 2 * 32 + 8 * 11 + 1 + 4 + 3 * 4);
+    b += T * (12 + 16 + 8);                // v2t, tri, elarea
+    b += E * (8 + 16 + 32 + 8);            // enode, eds, eflux, v2e
+    // patch tables: local triangles (shorts, number, area), Green-Gauss and
+    // residual term lists, local edges (ends, number, normal), halo,
+    // headers; bounded for patches down to 24 vertices
+    b += P * 640;
+    b += 6LL * 8 * kHistCap + 64 * 1024;
+    (void)iflow;
     return b;
 }
 
@@ -2459,10 +2465,26 @@ int fvm_batch_create(fvm_batch **out)
     return FVM_OK;
 }
 
+namespace {
+
+// waits for the launched chunk and hands each member its monitors
+int batch_wait(fvm_batch *b)
+{
+    if (!b->pending) return FVM_OK;
+    b->pending = false;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < b->mem.size(); ++i) b->mem[i]->hS = b->hs[i];
+    return FVM_OK;
+}
+
+}  // namespace
+
 int fvm_batch_add(fvm_batch *b, fvm_ctx *x)
 {
     if (!b || !x || x->batch || x->variant != 3) return FVM_E_INPUT;
     if (!b->mem.empty() && !same_kind(b->mem[0]->par, x->par)) return FVM_E_INPUT;
+    int rc = batch_wait(b);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(x->stream));
     b->mem.push_back(x);
     x->batch = b;
@@ -2473,7 +2495,8 @@ int fvm_batch_add(fvm_batch *b, fvm_ctx *x)
 int fvm_batch_remove(fvm_batch *b, fvm_ctx *x)
 {
     if (!b || !x || x->batch != b) return FVM_E_INPUT;
-    HIPCHK(hipStreamSynchronize(b->stream));
+    int rc = batch_wait(b);
+    if (rc) return rc;
     b->mem.erase(std::find(b->mem.begin(), b->mem.end(), x));
     x->batch = nullptr;
     b->dirty = true;
@@ -2482,53 +2505,68 @@ int fvm_batch_remove(fvm_batch *b, fvm_ctx *x)
 
 int fvm_batch_size(const fvm_batch *b) { return b ? (int)b->mem.size() : 0; }
 
-int fvm_batch_iterate(fvm_batch *b, int n, int *status)
+int fvm_batch_launch(fvm_batch *b, int n)
 {
     const int nm = (int)b->mem.size();
-    if (nm == 0 || n <= 0) return nm == 0 && n > 0 ? FVM_E_INPUT : FVM_OK;
+    if (n < 0 || n > kHistCap || b->pending) return FVM_E_INPUT;
+    if (nm == 0 || n == 0) return FVM_OK;
     if (b->dirty) {
         int rc = batch_rebuild(b);
         if (rc) return rc;
     }
-    int left = n;
     bool running = false;
     for (const fvm_ctx *x : b->mem) running = running || !x->hS.done;
-    while (left > 0 && running) {
-        const int chunk = std::min(left, kHistCap);
-        hipLaunchKernelGGL(k_begin_batch, dim3(nblk(nm)), dim3(kBlock), 0, b->stream, b->d_mem, nm);
-        if (b->graphs_ok && !b->graph_n) {
-            if (batch_graph(b, kGraphIters, &b->graph_n) || batch_graph(b, 1, &b->graph_1)) {
-                b->graphs_ok = false;
-                (void)hipGetLastError();
-            }
+    if (!running) return FVM_OK;
+    hipLaunchKernelGGL(k_begin_batch, dim3(nblk(nm)), dim3(kBlock), 0, b->stream, b->d_mem, nm);
+    if (b->graphs_ok && !b->graph_n) {
+        if (batch_graph(b, kGraphIters, &b->graph_n) || batch_graph(b, 1, &b->graph_1)) {
+            b->graphs_ok = false;
+            (void)hipGetLastError();
         }
-        if (b->graphs_ok) {
-            const int q = chunk / kGraphIters, r = chunk % kGraphIters;
-            for (int i = 0; i < q; ++i) HIPCHK(hipGraphLaunch(b->graph_n, b->stream));
-            for (int i = 0; i < r; ++i) HIPCHK(hipGraphLaunch(b->graph_1, b->stream));
-        } else {
-            for (int i = 0; i < chunk; ++i) batch_iteration(b);
-        }
-        hipLaunchKernelGGL(k_gather_batch, dim3(nblk(nm)), dim3(kBlock), 0, b->stream, b->d_mem, nm, b->d_states);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(b->hs.data(), b->d_states, sizeof(DevState) * nm, hipMemcpyDeviceToHost, b->stream));
-        HIPCHK(hipStreamSynchronize(b->stream));
-        running = false;
-        for (int i = 0; i < nm; ++i) {
-            b->mem[i]->hS = b->hs[i];
-            running = running || !b->hs[i].done;
-        }
-        left -= chunk;
     }
-    if (status)
-        for (int i = 0; i < nm; ++i) status[i] = member_status(b->mem[i]->hS);
+    if (b->graphs_ok) {
+        const int q = n / kGraphIters, r = n % kGraphIters;
+        for (int i = 0; i < q; ++i) HIPCHK(hipGraphLaunch(b->graph_n, b->stream));
+        for (int i = 0; i < r; ++i) HIPCHK(hipGraphLaunch(b->graph_1, b->stream));
+    } else {
+        for (int i = 0; i < n; ++i) batch_iteration(b);
+    }
+    hipLaunchKernelGGL(k_gather_batch, dim3(nblk(nm)), dim3(kBlock), 0, b->stream, b->d_mem, nm, b->d_states);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->hs.data(), b->d_states, sizeof(DevState) * nm, hipMemcpyDeviceToHost, b->stream));
+    b->pending = true;
     return FVM_OK;
+}
+
+int fvm_batch_wait(fvm_batch *b, int *status)
+{
+    int rc = batch_wait(b);
+    if (rc) return rc;
+    if (status)
+        for (size_t i = 0; i < b->mem.size(); ++i) status[i] = member_status(b->mem[i]->hS);
+    return FVM_OK;
+}
+
+int fvm_batch_iterate(fvm_batch *b, int n, int *status)
+{
+    if (b->mem.empty()) return n > 0 ? FVM_E_INPUT : FVM_OK;
+    int rc = batch_wait(b);
+    for (int left = n; !rc && left > 0; left -= kHistCap) {
+        bool running = false;
+        for (const fvm_ctx *x : b->mem) running = running || !x->hS.done;
+        if (!running) break;
+        rc = fvm_batch_launch(b, std::min(left, kHistCap));
+        if (!rc) rc = batch_wait(b);
+    }
+    if (!rc && status)
+        for (size_t i = 0; i < b->mem.size(); ++i) status[i] = member_status(b->mem[i]->hS);
+    return rc;
 }
 
 int fvm_batch_destroy(fvm_batch *b)
 {
     if (!b) return FVM_OK;
-    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    (void)batch_wait(b);
     for (fvm_ctx *x : b->mem) x->batch = nullptr;
     b->mem.clear();
     batch_free(b);

@@ -98,6 +98,10 @@ def lib():
     L.fvm_batch_size.argtypes = [vp]
     L.fvm_batch_iterate.restype = ci
     L.fvm_batch_iterate.argtypes = [vp, ci, vp]
+    L.fvm_batch_launch.restype = ci
+    L.fvm_batch_launch.argtypes = [vp, ci]
+    L.fvm_batch_wait.restype = ci
+    L.fvm_batch_wait.argtypes = [vp, vp]
     L.fvm_batch_destroy.restype = ci
     L.fvm_batch_destroy.argtypes = [vp]
     L.fvm_gpu_footprint.restype = ctypes.c_longlong

@@ -103,38 +103,40 @@ def radial_fractions(L: np.ndarray, n: int, d1: float, ratio: float | None) -> n
     """Normalised distances xi[i, j], j = 0..n, of the radial points on each
     line of length L[i]: xi[:, 0] = 0, xi[:, n] = 1."""
     L = np.asarray(L, dtype=np.float64)
+    if ratio is None:
+        # one ratio per line, all lines at once
+        r = _solve_ratio(d1, n, L)
+        h = d1 * r[:, None] ** np.arange(n)[None, :]
+        c = np.concatenate([np.zeros((L.size, 1)), np.cumsum(h, axis=1)], axis=1)
+        return c / c[:, -1:]
     xi = np.zeros((L.size, n + 1))
     for i, Li in enumerate(L):
-        if ratio is None:
-            r = float(_solve_ratio(d1, n, np.array([Li]))[0])
-            h = d1 * r ** np.arange(n)
-        else:
-            # wall cells at the given ratio, as many as leave room for the rest
-            # to grow (ratio >= 1) up to the far circle
-            m = 0
-            for k in range(1, n):
-                head = d1 * (ratio**k - 1.0) / (ratio - 1.0)
-                last = d1 * ratio ** (k - 1)
-                if head + (n - k) * last * ratio <= Li:
-                    m = k
-                else:
-                    break
-            head_h = d1 * ratio ** np.arange(m)
-            rest = Li - head_h.sum()
-            last = head_h[-1] if m > 0 else d1 / ratio
-            # spacing continues from the last wall cell with ratio r2:
-            # last*r2 + last*r2^2 + ... = rest
-            lo, hi = 1.0, ratio
-            for _ in range(200):
-                mid = 0.5 * (lo + hi)
-                tot = last * mid * (mid ** (n - m) - 1.0) / (mid - 1.0) if mid != 1.0 else last * (n - m)
-                if tot > rest:
-                    hi = mid
-                else:
-                    lo = mid
-            r2 = 0.5 * (lo + hi)
-            tail_h = last * r2 ** np.arange(1, n - m + 1)
-            h = np.concatenate([head_h, tail_h])
+        # wall cells at the given ratio, as many as leave room for the rest
+        # to grow (ratio >= 1) up to the far circle
+        m = 0
+        for k in range(1, n):
+            head = d1 * (ratio**k - 1.0) / (ratio - 1.0)
+            last = d1 * ratio ** (k - 1)
+            if head + (n - k) * last * ratio <= Li:
+                m = k
+            else:
+                break
+        head_h = d1 * ratio ** np.arange(m)
+        rest = Li - head_h.sum()
+        last = head_h[-1] if m > 0 else d1 / ratio
+        # spacing continues from the last wall cell with ratio r2:
+        # last*r2 + last*r2^2 + ... = rest
+        lo, hi = 1.0, ratio
+        for _ in range(200):
+            mid = 0.5 * (lo + hi)
+            tot = last * mid * (mid ** (n - m) - 1.0) / (mid - 1.0) if mid != 1.0 else last * (n - m)
+            if tot > rest:
+                hi = mid
+            else:
+                lo = mid
+        r2 = 0.5 * (lo + hi)
+        tail_h = last * r2 ** np.arange(1, n - m + 1)
+        h = np.concatenate([head_h, tail_h])
         c = np.concatenate([[0.0], np.cumsum(h)])
         xi[i] = c / c[-1]
     return xi

@@ -96,15 +96,7 @@ def geometric(coord, elem, ptype, cell_type: int = 1, iflow: int = 1,
     ``wall_distance="gpu"``, from fvm_gpu_wall_dist (same values)."""
     if iflow == 3 and wall_distance == "gpu":
         geo = geometric(coord, elem, ptype, cell_type, 1, sa_compat)
-        wd = np.zeros(geo.dims["np"])
-        wd1 = np.zeros(max(geo.dims["nsp"], 1))
-        rc = gpu.lib().fvm_gpu_wall_dist(geo.dims["np"], _p(geo.coord), _p(geo.ptype),
-                                         geo.dims["netot"], _p(geo.edge), sa_compat, _p(wd),
-                                         _p(wd1))
-        if rc != 0:
-            raise FvmError(rc, 0, "fvm_gpu_wall_dist")
-        geo.wd = wd
-        geo.wd1 = wd1[:geo.dims["nsp"]].copy()
+        gpu_wall_distance(geo, sa_compat)
         return geo
     L = gpu.lib()
     coord = np.ascontiguousarray(coord, dtype=np.float64).reshape(-1, 2)
@@ -139,6 +131,18 @@ def geometric(coord, elem, ptype, cell_type: int = 1, iflow: int = 1,
                     dsb=dsb, elarea=elarea, cvarea=cvarea, mcarea=mcarea, drmin=drmin, wd=wd,
                     wd1=wd1[:d["nsp"]].copy(), beindx=beindx[:d["nbe"]].copy(),
                     spts=spts[:d["nsp"]].copy())
+
+
+def gpu_wall_distance(geo: Geometry, sa_compat: int = 1) -> None:
+    """geo.wd, geo.wd1 from fvm_gpu_wall_dist (src/wall_dist.f)."""
+    wd = np.zeros(geo.dims["np"])
+    wd1 = np.zeros(max(geo.dims["nsp"], 1))
+    rc = gpu.lib().fvm_gpu_wall_dist(geo.dims["np"], _p(geo.coord), _p(geo.ptype),
+                                     geo.dims["netot"], _p(geo.edge), sa_compat, _p(wd), _p(wd1))
+    if rc != 0:
+        raise FvmError(rc, 0, "fvm_gpu_wall_dist")
+    geo.wd = wd
+    geo.wd1 = wd1[:geo.dims["nsp"]].copy()
 
 
 def mesh_struct(g: Geometry, keep: list) -> Mesh:
@@ -326,6 +330,20 @@ class Batch:
         rc = self.L.fvm_batch_iterate(self.h, n, _p(st))
         if rc != 0:
             raise FvmError(rc, 0, "fvm_batch_iterate")
+        return [int(v) for v in st[: len(self.members)]]
+
+    def launch(self, n: int):
+        """Enqueue up to n (<= 4096) iterations and return at once."""
+        rc = self.L.fvm_batch_launch(self.h, n)
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_batch_launch")
+
+    def wait(self) -> list[int]:
+        """Wait for the launched iterations; statuses in member order."""
+        st = np.zeros(max(len(self.members), 1), dtype=np.int32)
+        rc = self.L.fvm_batch_wait(self.h, _p(st))
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_batch_wait")
         return [int(v) for v in st[: len(self.members)]]
 
     def close(self):

