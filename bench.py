@@ -15,6 +15,7 @@ bounded sample of the same workload).
 from __future__ import annotations
 
 import argparse
+import faulthandler
 import json
 import os
 import platform
@@ -73,7 +74,91 @@ def cpu_baseline(grid, kw, target_s: float = 12.0) -> dict:
                       f"{platform.processor() or platform.machine()}, {dt:.1f} s"}
 
 
+def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
+    """Batches: (a) eight solves of the same grid at alpha = 0, 2, ..., 14 deg
+    in one launch per RK stage, FP64 and split precision; (b) 48 requests of
+    the serving workload's distribution (NACA 4-digit airfoils, 1k-20k
+    vertices, 70 % Euler / 30 % SA-RANS, seeded) as two batches (one per flow
+    kind, one stream each), split precision, against the same requests
+    solved one after another by the plain FP64 path."""
+    from fvm2d_amd import serving
+    from fvm2d_amd.solver import Batch, Inputs, Solver
+    out = {}
+    steps = 200
+    alphas = [2.0 * k for k in range(8)]
+    for prec, name in ((0, "fp64"), (1, "split")):
+        sol = [Solver(g, Inputs.from_kwargs(**dict(kw, aoa_deg=al, precision=prec)), geometry=geo)
+               for al in alphas]
+        with Batch() as b:
+            for x in sol:
+                b.add(x)
+            b.iterate(warmup)
+            t0 = time.perf_counter()
+            b.iterate(steps)
+            dt = time.perf_counter() - t0
+        out[f"same_grid_8_alphas_{name}"] = round(len(sol) * geo.dims["np"] * steps / dt, 1)
+        for x in sol:
+            x.close()
+    out["same_grid_8_alphas_split_vs_plain_fp64"] = round(out["same_grid_8_alphas_split"] / plain_value, 2)
+
+    reqs = serving.workload(48, 2024, np_min=1e3, np_max=2e4)
+    cfg = serving.ServeConfig(max_iters=10**6, minres=0.0)
+    prepared = [(q, *serving.prepare(q, cfg)) for q in reqs]
+    # plain FP64 path, one request at a time (iteration time only)
+    c64 = serving.ServeConfig(max_iters=10**6, minres=0.0, precision=0)
+    vi = 0.0
+    tp = 0.0
+    from fvm2d_amd.gpu import FvmError
+    for q, g1, geo1 in prepared:
+        with serving.create(q, g1, geo1, c64) as x:
+            x.iterate(8)
+            i0 = x.monitor()[0]["iter"]
+            t0 = time.perf_counter()
+            try:
+                x.iterate(steps)
+            except FvmError:            # a solve that stops on a reference STOP condition
+                pass
+            tp += time.perf_counter() - t0
+            vi += x.np * (x.monitor()[0]["iter"] - i0)
+    plain = vi / tp
+    # batched split precision, one batch per flow kind, chunks launched together
+    sol = {"euler": [], "rans": []}
+    for q, g1, geo1 in prepared:
+        sol[q.flow].append(serving.create(q, g1, geo1, cfg))
+    batches = []
+    for xs in sol.values():
+        if xs:
+            b = Batch()
+            for x in xs:
+                b.add(x)
+            batches.append(b)
+    for b in batches:
+        b.iterate(8)
+    i0 = {id(x): x.monitor()[0]["iter"] for xs in sol.values() for x in xs}
+    t0 = time.perf_counter()
+    for _ in range(steps // 50):
+        for b in batches:
+            b.launch(50)
+        for b in batches:
+            b.wait()
+    dt = time.perf_counter() - t0
+    nv = sum(x.np for xs in sol.values() for x in xs)
+    vb = sum(x.np * (x.monitor()[0]["iter"] - i0[id(x)]) for xs in sol.values() for x in xs)
+    for b in batches:
+        b.close()
+    for xs in sol.values():
+        for x in xs:
+            x.close()
+    out["mixed_48_requests"] = {
+        "workload": "48 NACA 4-digit requests, 1k-20k vertices log-uniform, 70/30 Euler/SA-RANS, "
+                    f"{nv} vertices in all, {steps} iterations each",
+        "batched_split": round(vb / dt, 1), "plain_fp64_one_at_a_time": round(plain, 1),
+        "speedup": round(vb / dt / plain, 2)}
+    return out
+
+
 def main() -> int:
+    faulthandler.enable()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
@@ -81,6 +166,7 @@ def main() -> int:
     ap.add_argument("--flux", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the split-precision and batched legs")
     ap.add_argument("--variant", type=int, default=None, help="kernel layout 1 or 2 (default: tuned)")
     ap.add_argument("--vb", type=int, default=64, help="vertices per workgroup (variant 2)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -140,6 +226,20 @@ def main() -> int:
                         "kernel_ms": {k: round(v, 5) for k, v in ms.items()},
                         "algorithmic_bytes_per_launch": B}
 
+    # split precision on the same workload: CL/CD against this FP64 run
+    split = None
+    if not a.no_batch:
+        with Solver(g, Inputs.from_kwargs(precision=1, **kw), geometry=s.geo) as s1:
+            s1.iterate(a.warmup)
+            t1 = time.perf_counter()
+            h1, _ = s1.iterate(a.steps)
+            e1 = time.perf_counter() - t1
+        split = {"value": round(npt * a.steps / e1, 1), "unit": "vertex-iterations/s",
+                 "dcl": abs(float(h1[-1, 4] - hist[-1, 4])), "dcd": abs(float(h1[-1, 5] - hist[-1, 5])),
+                 "note": "same workload and step count with FP32 reconstruction and flux; "
+                         "dcl, dcd against this FP64 run"}
+    batched = None if a.no_batch else batch_legs(g, s.geo, kw, value, a.warmup)
+
     cpu = None if a.no_cpu else cpu_baseline(g, kw)
     line = {
         "metric": "vertex-iters/s", "value": round(value, 1), "unit": "vertex-iterations/s",
@@ -150,7 +250,7 @@ def main() -> int:
                                "Euler, %s flux, limited MUSCL, RK3, CFL 1, M=0.8, alpha=1.25, FP64"
                                % {1: "Roe", 2: "KFVS", 3: "HLLC"}[a.flux],
                    "np": npt, "nt": nt, "ne": ne, "precision": "fp64"},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "split_precision": split, "batched": batched,
         "cl": float(hist[-1, 4]), "cd": float(hist[-1, 5]),
     }
     if cpu:
