@@ -24,6 +24,9 @@ ARCH = os.environ.get("FVM2D_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 CXX = shutil.which("g++") or "g++"
+FLANG = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+FORTRAN = os.path.join(HERE, "fortran")
+HOST_EXE = os.path.join(OUTDIR, "fvm2d_gpu")
 
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -53,7 +56,27 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    if os.path.exists(FLANG):
+        build_host(verbose)
     return LIB
+
+
+def build_host(verbose: bool = False) -> str:
+    """The Fortran host program (fortran/fvm2d_gpu.f90) linked against the
+    library, found next to it at run time."""
+    mod = os.path.join(OUTDIR, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    objs = []
+    for src in ("fvm_gpu_mod.f90", "fvm2d_gpu.f90"):
+        obj = os.path.join(mod, src.replace(".f90", ".o"))
+        _run([FLANG, "-O2", "-J", mod, "-c", os.path.join(FORTRAN, src), "-o", obj])
+        objs.append(obj)
+    tmp = HOST_EXE + ".tmp"
+    _run([FLANG, "-o", tmp, *objs, "-L", OUTDIR, "-lfvm2d_gpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_EXE)
+    if verbose:
+        print("built", HOST_EXE)
+    return HOST_EXE
 
 
 if __name__ == "__main__":

@@ -1,0 +1,133 @@
+"""The Fortran host program (fvm2d_amd/fortran/fvm2d_gpu.f90): the
+reference's deck, grid and SOL files in, RES.DAT / WALL.DAT / SOL /
+PARAM.DAT out, the iterations through the C-ABI on the GPU.  Its outputs are
+compared with the reference's own values (golden fixtures): RES.DAT to the
+six digits it prints, SOL to the digits the list-directed write keeps."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fvm2d_amd import build, mesh
+from fvm2d_amd.solver import geometric
+from tests.golden.cases import CASES, deck_text
+from tests.golden.make_golden import write_sol
+from tests.helpers import fixture, grid
+
+pytestmark = pytest.mark.gpu
+
+GAMMA1 = 0.4
+
+
+def conserved(prim, nut):
+    r, u, v, p = np.asarray(prim).reshape(-1, 4).T
+    return np.stack([r, r * u, r * v, p / GAMMA1 + 0.5 * r * (u**2 + v**2), nut], axis=1)
+
+
+def run_host(tmp_path, name, save_every=None, precision=0):
+    case = CASES[name]
+    d = dict(case["deck"])
+    d["MaxIter"] = case["iters"]
+    if save_every:
+        d["SaveInt"] = save_every
+    mesh.write_amdba(str(tmp_path / "grid.amdba"), grid(name))
+    (tmp_path / "fvm.inp").write_text(deck_text(d))
+    fx = fixture(name)
+    if d["Start"] == 2:
+        write_sol(str(tmp_path / "SOL"), fx["prim0"], fx["nut0"])
+    assert os.path.exists(build.HOST_EXE), "build the host program first (python -m fvm2d_amd.build)"
+    r = subprocess.run([build.HOST_EXE, "fvm.inp", str(precision)], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return fx, r.stdout
+
+
+def read_list_directed(path, ncol):
+    """A file of list-directed writes of ncol numbers each (records may
+    wrap; the reference reads SOL back the same way, src/initialize.f:95)."""
+    vals = np.array([float(x) for x in open(path).read().split()])
+    return vals.reshape(-1, ncol)
+
+
+def read_sol(path):
+    return read_list_directed(path, 5)
+
+
+def read_res(path):
+    rows = [ln.split() for ln in open(path) if ln.strip()]
+    return np.array([[float(x) for x in row] for row in rows])
+
+
+def fortran_e14_6(v: float) -> str:
+    """Fortran's E14.6 edit descriptor: 0.dddddd with a two-digit exponent,
+    right-aligned in 14 columns (six significant digits, rounded)."""
+    if v == 0.0:
+        return f"{'0.000000E+00':>14}"
+    m, e = f"{abs(v):.5E}".split("E")
+    e = int(e) + 1
+    exp = f"E{e:+03d}" if abs(e) <= 99 else f"{e:+04d}"     # 3-digit exponents drop the E
+    s = f"{'-' if v < 0 else ''}0.{m.replace('.', '')}{exp}"
+    return f"{s:>14}"
+
+
+def res_line(row) -> str:
+    """A RES.DAT row, format (I8,E14.6,I8,3E14.6) of src/main.f:63."""
+    return (f"{int(row[0]):8d}{fortran_e14_6(row[1])}{int(row[2]):8d}"
+            + "".join(fortran_e14_6(x) for x in row[3:6]))
+
+
+@pytest.mark.parametrize("name", ["euler_roe_transonic_65x17", "restart_roe_nolim_129x33",
+                                  "laminar_roe_97x25", "rans_roe_97x25"])
+def test_host_program_matches_reference(tmp_path, name):
+    fx, out = run_host(tmp_path, name, save_every=10)
+    res = read_res(tmp_path / "RES.DAT")
+    ref = fx["hist"]
+    assert res.shape == ref.shape
+    np.testing.assert_array_equal(res[:, 0], ref[:, 0])                 # iteration numbers
+    np.testing.assert_array_equal(res[:, 2], ref[:, 2])                 # iresi
+    if name in ("euler_roe_transonic_65x17", "restart_roe_nolim_129x33"):
+        # bitwise paths: the text the reference's write statement gives its values
+        lines = [ln.rstrip("\n") for ln in open(tmp_path / "RES.DAT") if ln.strip()]
+        assert lines == [res_line(r) for r in ref]
+    for col in (1, 3, 4, 5):                                            # E14.6: 6 digits
+        assert np.all(np.abs(res[:, col] - ref[:, col]) <= 5e-6 * np.abs(ref[:, col]) + 1e-300)
+    sol = read_sol(tmp_path / "SOL")
+    want = conserved(fx["primN"], fx["nutN"] if "nutN" in fx else np.zeros(sol.shape[0]))
+    tol = 1e-14 if name in ("euler_roe_transonic_65x17", "restart_roe_nolim_129x33") else 1e-8
+    assert np.abs(sol - want).max() <= tol * np.abs(want).max()
+    ncol = 2 if CASES[name]["deck"]["FlowType"] == 1 else 5
+    wall = read_list_directed(tmp_path / "WALL.DAT", ncol)
+    g = grid(name)
+    assert wall.shape[0] == int((g.ptype == 3).sum())
+    # Cp at the solid points (src/result.f:96) from the reference's state
+    p_inf = 1.0 / (1.4 * CASES[name]["deck"]["Mach"] ** 2)
+    prim = np.asarray(fx["primN"]).reshape(-1, 4)
+    spts = geometric(g.coord, g.elem, g.ptype).spts - 1          # WALL.DAT row order
+    np.testing.assert_array_equal(wall[:, 0], g.coord[spts, 0])
+    cp = -(prim[spts, 3] - p_inf) / 0.5
+    assert np.abs(wall[:, 1] - cp).max() <= 1e-8 * max(1.0, np.abs(cp).max())
+    assert "Stopping execution" in out
+    assert (tmp_path / "PARAM.DAT").exists()
+
+
+def test_host_program_restarts_from_its_own_sol(tmp_path):
+    """Two runs of 50 iterations with a restart in between give the state of
+    one run of 100.  SOL keeps every digit of the conserved variables; the
+    primitive state rebuilt from them differs in the last bits, hence 1e-9."""
+    name = "euler_roe_transonic_65x17"
+    fx, _ = run_host(tmp_path, name)
+    one = read_sol(tmp_path / "SOL")
+    d = dict(CASES[name]["deck"])
+    d.update(MaxIter=50)
+    (tmp_path / "fvm.inp").write_text(deck_text(d))
+    subprocess.run([build.HOST_EXE, "fvm.inp"], cwd=tmp_path, check=True, capture_output=True,
+                   timeout=120)
+    d.update(Start=2, LastIter=50, MaxIter=50)        # MaxIter counts from LastIter
+    (tmp_path / "fvm.inp").write_text(deck_text(d))
+    subprocess.run([build.HOST_EXE, "fvm.inp"], cwd=tmp_path, check=True, capture_output=True,
+                   timeout=120)
+    two = read_sol(tmp_path / "SOL")
+    assert np.abs(two - one).max() <= 1e-9 * np.abs(one).max()
+    res = read_res(tmp_path / "RES.DAT")
+    assert res[0, 0] == 50 and res[-1, 0] == 99

@@ -1,6 +1,9 @@
 """The C-ABI library loads and exports every entry point include/fvm2d_gpu.h
 declares (no GPU needed: nothing here launches a kernel)."""
 import ctypes
+import os
+
+import pytest
 
 from fvm2d_amd import gpu
 
@@ -30,3 +33,15 @@ def test_footprint_estimate_scales():
     b = L.fvm_gpu_footprint_estimate(2 * 66048, 2 * 131072, 2 * 197120, 1, 0)
     assert 1.9 < b / a < 2.1
     assert 200 < a / 66048 < 2000
+
+
+def test_host_program_built():
+    """The Fortran host program links against the library and runs its
+    argument check without a GPU."""
+    import subprocess
+    from fvm2d_amd import build
+    if not os.path.exists(build.FLANG):
+        pytest.skip("no flang")
+    assert os.path.exists(build.HOST_EXE)
+    r = subprocess.run([build.HOST_EXE], capture_output=True, text=True, timeout=60)
+    assert "You must specify an input file." in r.stdout
