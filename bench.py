@@ -157,6 +157,73 @@ def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
     return out
 
 
+def config_legs(iters_lam: int, iters_rans: int) -> dict:
+    """Split precision against FP64 on the viscous workloads.
+
+    Laminar: NACA0012 1025x257 O-grid (wall spacing 1e-4, ratio 1.1), M 0.5,
+    alpha 0, Re 5000, Roe, limited.  RANS: NACA0012 2049x513 O-grid (wall
+    spacing 4e-6, ratio 1.15), M 0.15, Re 6e6, SA with freestream nu_tilde =
+    3 nu, a batch of 8 solves at alpha = 0, 2, ..., 14 deg in one launch per
+    RK stage.  Both run a bounded number of iterations from the free stream
+    (stated), FP64 and split; the differences are taken at the same
+    iteration: CL, CD and the L2 density-residual history."""
+    import numpy as np
+    from fvm2d_amd import mesh
+    from fvm2d_amd.solver import Batch, Inputs, Solver, geometric
+    out = {}
+    g = mesh.naca0012_1025x257()
+    geo = geometric(g.coord, g.elem, g.ptype, iflow=2)
+    kw = dict(iflow=2, iflux=1, ilimit=1, mach_inf=0.5, aoa_deg=0.0, rey=5000.0, cfl=1.0,
+              maxiter=10**9, minres=0.0)
+    lam = {}
+    for prec in (0, 1):
+        print(f"laminar 1025x257, precision {prec}", file=sys.stderr, flush=True)
+        with Solver(g, Inputs.from_kwargs(precision=prec, **kw), geometry=geo) as x:
+            x.iterate(2)
+            t0 = time.perf_counter()
+            h, _ = x.iterate(iters_lam)
+            lam[prec] = (h, np.float64(geo.dims["np"]) * iters_lam / (time.perf_counter() - t0))
+    h0, h1 = lam[0][0], lam[1][0]
+    out["laminar_1025x257"] = {
+        "iterations": iters_lam, "np": geo.dims["np"],
+        "fp64_vertex_iters_per_s": round(float(lam[0][1]), 1),
+        "split_vertex_iters_per_s": round(float(lam[1][1]), 1),
+        "cl_fp64": float(h0[-1, 4]), "cd_fp64": float(h0[-1, 5]),
+        "dcl": abs(float(h1[-1, 4] - h0[-1, 4])), "dcd": abs(float(h1[-1, 5] - h0[-1, 5])),
+        "max_rel_dfres": float(np.max(np.abs(h1[:, 1] - h0[:, 1]) / np.abs(h0[:, 1])))}
+    del geo
+    g = mesh.naca0012_2049x513()
+    geo = geometric(g.coord, g.elem, g.ptype, iflow=3, wall_distance="gpu")
+    kw = dict(iflow=3, iflux=1, ilimit=1, mach_inf=0.15, rey=6.0e6, cfl=1.0, maxiter=10**9,
+              minres=0.0, nut_factor=3.0)
+    alphas = [2.0 * k for k in range(8)]
+    rans = {}
+    for prec in (0, 1):
+        print(f"RANS 2049x513 batch of 8, precision {prec}", file=sys.stderr, flush=True)
+        sol = [Solver(g, Inputs.from_kwargs(precision=prec, aoa_deg=al, **kw), geometry=geo)
+               for al in alphas]
+        with Batch() as b:
+            for x in sol:
+                b.add(x)
+            b.iterate(2)
+            t0 = time.perf_counter()
+            st = b.iterate(iters_rans)
+            dt = time.perf_counter() - t0
+            mons = [x.monitor()[0] for x in sol]
+        for x in sol:
+            x.close()
+        rans[prec] = (mons, len(sol) * geo.dims["np"] * iters_rans / dt, st)
+    out["rans_2049x513_batch8"] = {
+        "iterations": iters_rans, "np": geo.dims["np"], "alphas_deg": alphas,
+        "fp64_vertex_iters_per_s": round(rans[0][1], 1),
+        "split_vertex_iters_per_s": round(rans[1][1], 1),
+        "status_fp64": rans[0][2], "status_split": rans[1][2],
+        "cl_fp64": [m["cl"] for m in rans[0][0]], "cd_fp64": [m["cd"] for m in rans[0][0]],
+        "dcl": [abs(a["cl"] - b["cl"]) for a, b in zip(rans[0][0], rans[1][0])],
+        "dcd": [abs(a["cd"] - b["cd"]) for a, b in zip(rans[0][0], rans[1][0])]}
+    return out
+
+
 def main() -> int:
     faulthandler.enable()
     ap = argparse.ArgumentParser()
@@ -169,6 +236,9 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the split-precision and batched legs")
     ap.add_argument("--variant", type=int, default=None, help="kernel layout 1 or 2 (default: tuned)")
     ap.add_argument("--vb", type=int, default=64, help="vertices per workgroup (variant 2)")
+    ap.add_argument("--configs", nargs=2, type=int, default=None, metavar=("LAM", "RANS"),
+                    help="only run the split-vs-FP64 legs of the laminar 1025x257 and RANS "
+                         "2049x513 batch-of-8 workloads for LAM and RANS iterations")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES, five times: the "
                          "reference reading for the FETCH_SIZE/WRITE_SIZE counters")
@@ -182,6 +252,9 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": 4 * n, "copies": 5}))
+        return 0
+    if a.configs:
+        print(json.dumps(config_legs(*a.configs)), flush=True)
         return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the workload is a single-GPU solve and this build "

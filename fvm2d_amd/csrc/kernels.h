@@ -53,6 +53,21 @@ __device__ __forceinline__ V4 to_d(const V4f &a)
 // type: T = double is the reference's arithmetic, T = float the split-
 // precision mode (geometry, accumulation, update and norms stay FP64).
 
+// Division, square root, exp and erf of the reconstruction and fluxes.  In
+// FP64 they are the IEEE operations (and libm), as in the reference.  In the
+// split-precision mode (T = float) they are the hardware approximations
+// (v_rcp_f32 times the numerator, v_sqrt_f32, fast exp): about 1 ulp of
+// FP32, well inside what the FP32 arithmetic around them already loses, at
+// a fraction of the instructions of correctly rounded FP32 division.
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double fsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ double fexp(double a) { return exp(a); }
+__device__ __forceinline__ float fexp(float a) { return __expf(a); }
+__device__ __forceinline__ double ferf(double a) { return erf(a); }
+__device__ __forceinline__ float ferf(float a) { return erff(a); }
+
 // van Albada limiter, src/recon.f:55-71
 template <class T>
 __device__ __forceinline__ T limit_albada(T ul, T ur)
@@ -60,7 +75,7 @@ __device__ __forceinline__ T limit_albada(T ul, T ur)
     if (ul * ur <= T(0)) return T(0);
     T top = (ul * ul + T(kEps)) * ur + (ur * ur + T(kEps)) * ul;
     T bot = ul * ul + ur * ur + T(2.0 * kEps);
-    return top / bot;
+    return fdiv(top, bot);
 }
 
 // MUSCL states at the edge midpoint, src/recon.f:15-44
@@ -93,30 +108,30 @@ __device__ __forceinline__ bool roe_flux(const Consts &c, T dsx, T dsy, const VT
                                          const VT<T> &R, VT<T> &f)
 {
     const T G = T(c.GAMMA), G1 = T(c.GAMMA1), h = T(0.5);
-    T length = sqrt(dsx * dsx + dsy * dsy);
-    T ct = dsx / length, st = dsy / length;
+    T length = fsqrt(dsx * dsx + dsy * dsy);
+    T ct = fdiv(dsx, length), st = fdiv(dsy, length);
     T rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
-    T al2 = G * pl / rl, ql2 = ul * ul + vl * vl, hl = al2 / G1 + h * ql2;
+    T al2 = fdiv(G * pl, rl), ql2 = ul * ul + vl * vl, hl = fdiv(al2, G1) + h * ql2;
     T rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
-    T ar2 = G * pr / rr, qr2 = ur * ur + vr * vr, hr = ar2 / G1 + h * qr2;
+    T ar2 = fdiv(G * pr, rr), qr2 = ur * ur + vr * vr, hr = fdiv(ar2, G1) + h * qr2;
     T unl = ul * ct + vl * st;
-    T rl12 = sqrt(rl), rr12 = sqrt(rr), rd = T(1) / (rl12 + rr12);
+    T rl12 = fsqrt(rl), rr12 = fsqrt(rr), rd = fdiv(T(1), rl12 + rr12);
     T ua = (ul * rl12 + ur * rr12) * rd;
     T va = (vl * rl12 + vr * rr12) * rd;
     T ha = (hl * rl12 + hr * rr12) * rd;
     T qa2 = ua * ua + va * va;
     T aa2 = G1 * (ha - h * qa2);
     if (!(aa2 > T(0))) return false;
-    T aa = sqrt(aa2);
+    T aa = fsqrt(aa2);
     T una = ua * ct + va * st;
     T vna = -ua * st + va * ct;
     T l1 = fmin(una - aa, T(0)), l2 = fmin(una, T(0)), l4 = fmin(una + aa, T(0));
     T du1 = rr - rl, du2 = rr * ur - rl * ul, du3 = rr * vr - rl * vl;
-    T du4 = (pr - pl) / G1 + h * (rr * qr2 - rl * ql2);
-    T m1 = (ct * du2 + st * du3 - una * du1) / aa;
-    T m2 = G1 * (du4 - ua * du2 - va * du3 + qa2 * du1) / (aa * aa);
+    T du4 = fdiv(pr - pl, G1) + h * (rr * qr2 - rl * ql2);
+    T m1 = fdiv(ct * du2 + st * du3 - una * du1, aa);
+    T m2 = fdiv(G1 * (du4 - ua * du2 - va * du3 + qa2 * du1), aa * aa);
     T a4 = h * (m1 + m2), a1 = h * (m2 - m1), a3 = du1 - a1 - a4;
-    T a2 = (st * du2 - ct * du3 + vna * du1) / aa;
+    T a2 = fdiv(st * du2 - ct * du3 + vna * du1, aa);
     T a1l1 = a1 * l1, a2l2 = a2 * l2, a3l3 = a3 * l2, a4l4 = a4 * l4;
     T aact = aa * ct, aast = aa * st;
     T Fc0 = rl * unl, Fc1 = pl * ct + rl * ul * unl, Fc2 = pl * st + rl * vl * unl,
@@ -138,20 +153,20 @@ __device__ __forceinline__ void kfvs_flux(const Consts &c, T dsx, T dsy, const V
                                           const VT<T> &R, VT<T> &f)
 {
     const T G1 = T(c.GAMMA1), h = T(0.5), PI = T(c.M_PI_);
-    T length = sqrt(dsx * dsx + dsy * dsy);
-    T ct = dsx / length, st = dsy / length;
+    T length = fsqrt(dsx * dsx + dsy * dsy);
+    T ct = fdiv(dsx, length), st = fdiv(dsy, length);
     T rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
-    T el = pl / G1 + h * rl * (ul * ul + vl * vl);
+    T el = fdiv(pl, G1) + h * rl * (ul * ul + vl * vl);
     T rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
-    T er = pr / G1 + h * rr * (ur * ur + vr * vr);
+    T er = fdiv(pr, G1) + h * rr * (ur * ur + vr * vr);
     T unl = ul * ct + vl * st, unr = ur * ct + vr * st;
     T vnl = -ul * st + vl * ct, vnr = -ur * st + vr * ct;
-    T betal = h * rl / pl, sl = unl * sqrt(betal);
-    T Al = h * (T(1) + erf(sl));
-    T Bl = h * exp(-(sl * sl)) / sqrt(PI * betal);
-    T betar = h * rr / pr, sr = unr * sqrt(betar);
-    T Ar = h * (T(1) - erf(sr));
-    T Br = h * exp(-(sr * sr)) / sqrt(PI * betar);
+    T betal = fdiv(h * rl, pl), sl = unl * fsqrt(betal);
+    T Al = h * (T(1) + ferf(sl));
+    T Bl = fdiv(h * fexp(-(sl * sl)), fsqrt(PI * betal));
+    T betar = fdiv(h * rr, pr), sr = unr * fsqrt(betar);
+    T Ar = h * (T(1) - ferf(sr));
+    T Br = fdiv(h * fexp(-(sr * sr)), fsqrt(PI * betar));
     T F0 = rl * (unl * Al + Bl) + rr * (unr * Ar - Br);
     T F1 = ((pl + rl * (unl * unl)) * Al + rl * unl * Bl) + ((pr + rr * (unr * unr)) * Ar - rr * unr * Br);
     T F2 = rl * (unl * Al + Bl) * vnl + rr * (unr * Ar - Br) * vnr;
