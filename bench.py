@@ -236,6 +236,9 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the split-precision and batched legs")
     ap.add_argument("--variant", type=int, default=None, help="kernel layout 1 or 2 (default: tuned)")
     ap.add_argument("--vb", type=int, default=64, help="vertices per workgroup (variant 2)")
+    ap.add_argument("--tune", action="store_true",
+                    help="pick the launch shape with the autotuner (cached) instead of the "
+                         "occupancy model")
     ap.add_argument("--configs", nargs=2, type=int, default=None, metavar=("LAM", "RANS"),
                     help="only run the split-vs-FP64 legs of the laminar 1025x257 and RANS "
                          "2049x513 batch-of-8 workloads for LAM and RANS iterations")
@@ -268,9 +271,23 @@ def main() -> int:
     g = mesh.naca0012_513x129()
     kw = dict(iflow=1, iflux=a.flux, ilimit=1, mach_inf=0.8, aoa_deg=1.25, cfl=1.0,
               maxiter=10**9, minres=0.0)
+    launch = {"patch_size": 0, "xcd": 1, "chosen_by": "occupancy model"}
+    if a.tune:
+        from fvm2d_amd import autotune
+        from fvm2d_amd.solver import geometric
+        geo0 = geometric(g.coord, g.elem, g.ptype)
+        ch = autotune.tune(g, geo0, Inputs.from_kwargs(**kw), device="gfx950")
+        kw["patch_size"] = ch.patch_size
+        launch = {"patch_size": ch.patch_size, "xcd": ch.xcd, "chosen_by": "autotuner",
+                  "us_per_iter_tuning": round(ch.us_per_iter, 2)}
     s = Solver(g, Inputs.from_kwargs(**kw))
+    if a.tune:
+        s.tune(3, 64, launch["xcd"])
     if a.variant is not None:
         s.tune(a.variant, a.vb, 1)
+    info = s.layout_info()
+    launch.update(patches=info["patches"], patch_size=info["patch_size"],
+                  lds_bytes=info["lds_bytes"], workgroups_per_cu=info["workgroups_per_cu"])
     dims = s.geo.dims
     npt, nt, ne = dims["np"], dims["nt"], dims["ne"]
 
@@ -313,7 +330,7 @@ def main() -> int:
                          "dcl, dcd against this FP64 run"}
     batched = None if a.no_batch else batch_legs(g, s.geo, kw, value, a.warmup)
 
-    cpu = None if a.no_cpu else cpu_baseline(g, kw)
+    cpu = None if a.no_cpu else cpu_baseline(g, {k: v for k, v in kw.items() if k != "patch_size"})
     line = {
         "metric": "vertex-iters/s", "value": round(value, 1), "unit": "vertex-iterations/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -322,7 +339,7 @@ def main() -> int:
         "config": {"workload": "NACA0012 O-grid 513x129 (66048 vertices, 131072 triangles), "
                                "Euler, %s flux, limited MUSCL, RK3, CFL 1, M=0.8, alpha=1.25, FP64"
                                % {1: "Roe", 2: "KFVS", 3: "HLLC"}[a.flux],
-                   "np": npt, "nt": nt, "ne": ne, "precision": "fp64"},
+                   "np": npt, "nt": nt, "ne": ne, "precision": "fp64", "launch": launch},
         "roofline": roofline, "cpu_baseline": cpu, "split_precision": split, "batched": batched,
         "cl": float(hist[-1, 4]), "cd": float(hist[-1, 5]),
     }

@@ -75,6 +75,8 @@ struct Dev {
     int vb, nblocks, xcd;             // (layout 1) vertices per workgroup; XCD remap
     const int *orig;                  // 1-based vertex number in the caller's order
     double4 *uout;                    // far-field state of each farfield edge (conserved)
+    double4 *wall;                    // per solid-edge end point: p, txx, txy, tyy (last stage)
+    const double2 *wall_s;            // per solid edge: (dy, -dx) of src/clcd.f:23-24
 };
 
 
@@ -312,6 +314,26 @@ __device__ double2 block_sum2(double2 x)
     return r;
 }
 
+// What src/clcd.f:26-34 reads at a solid point, left by the last stage for
+// every solid edge the point ends: the new pressure and, with viscosity, the
+// stress of the last residual's gradients.  k_final then adds the edges in
+// the reference's order without gathering anything else.
+template <bool VISC>
+__device__ __forceinline__ void wall_terms(const Dev &d, int v, double p, double mu, double4 qx,
+                                           double4 qy)
+{
+    double4 w = make_double4(p, 0.0, 0.0, 0.0);
+    if (VISC) {
+        w.y = 2.0 / 3.0 * mu * (2.0 * qx.y - qy.z);
+        w.z = mu * (qy.y + qx.z);
+        w.w = 2.0 / 3.0 * mu * (2.0 * qy.z - qx.y);
+    }
+    for (int k = d.v2b_ptr[v]; k < d.v2b_ptr[v + 1]; ++k) {
+        const int q = d.v2b[k];
+        if ((q >> 1) < d.nsw) d.wall[q] = w;
+    }
+}
+
 // RK stage, src/update.f:17-35 (+ src/convert.f).  LAST: the third stage,
 // which also produces the conres partials and, for non-SA flow, the
 // positivity check (src/check_positivity.f:11-26).
@@ -338,6 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_update(Dev d, Consts c, DevState *S)
         q.w = c.GAMMA1 * (un.v[3] - 0.5 * (sq(un.v[1]) + sq(un.v[2])) / un.v[0]);
         d.prim[v] = q;
         if (LAST) {
+            if (d.solid[v]) wall_terms<VISC>(d, v, q.w, VISC ? d.mu[v] : 0.0, d.qx[v], d.qy[v]);
             double gra = q.x - po.v[0];
             double dif = fabs(gra);
             red.s = sq(gra);
@@ -482,25 +505,16 @@ __device__ __forceinline__ void final_body(const Dev &d, const Consts &c, DevSta
 #pragma unroll 2
     for (int k = t; k < d.nparts; k += kFinal)
         acc = red_join(acc, Red{d.partial[3 * k], d.partial[3 * k + 1], (int)d.partial[3 * k + 2]});
-    // forces on the solid edges
+    // forces on the solid edges, src/clcd.f:18-43
     double xf = 0.0, yf = 0.0;
 #pragma unroll 1
     for (int b = t; b < d.nsw; b += kFinal) {
-        int2 e = d.bnode[b];
-        double2 x1 = d.coord[e.x], x2 = d.coord[e.y];
-        double sx = x2.y - x1.y, sy = -(x2.x - x1.x);
-        double p1 = d.prim[e.x].w, p2 = d.prim[e.y].w;
-        double p = 0.5 * (p1 + p2);
+        const double4 w1 = d.wall[2 * b], w2 = d.wall[2 * b + 1];
+        const double2 sn = d.wall_s[b];
+        const double sx = sn.x, sy = sn.y;
+        double p = 0.5 * (w1.x + w2.x);
         if (VISC) {
-            double mu1 = d.mu[e.x], mu2 = d.mu[e.y];
-            double4 qx1 = d.qx[e.x], qy1 = d.qy[e.x], qx2 = d.qx[e.y], qy2 = d.qy[e.y];
-            double txx1 = 2.0 / 3.0 * mu1 * (2.0 * qx1.y - qy1.z);
-            double txy1 = mu1 * (qy1.y + qx1.z);
-            double tyy1 = 2.0 / 3.0 * mu1 * (2.0 * qy1.z - qx1.y);
-            double txx2 = 2.0 / 3.0 * mu2 * (2.0 * qx2.y - qy2.z);
-            double txy2 = mu2 * (qy2.y + qx2.z);
-            double tyy2 = 2.0 / 3.0 * mu2 * (2.0 * qy2.z - qx2.y);
-            double txx = 0.5 * (txx1 + txx2), txy = 0.5 * (txy1 + txy2), tyy = 0.5 * (tyy1 + tyy2);
+            double txx = 0.5 * (w1.y + w2.y), txy = 0.5 * (w1.z + w2.z), tyy = 0.5 * (w1.w + w2.w);
             xf = xf + p * sx - txx * sx - txy * sy;
             yf = yf + p * sy - txy * sx - tyy * sy;
         } else {
@@ -632,7 +646,9 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             double nu = d.nut[gv];
             double m = TURB ? ml + mu_turb(c, ml, p.x, nu) : ml;
             s_mul[i] = ml; s_mu[i] = m; s_nut[i] = nu;
-            if (i < NP) { d.mul[gv] = ml; d.mu[gv] = m; }
+            // mu, mul, qx, qy of the last residual of an iteration are what
+            // the monitors, the SA step and the caller read
+            if (MODE != 0 && i < NP) { d.mul[gv] = ml; d.mu[gv] = m; }
         }
     }
     for (int i = t; i <= NG; i += kBlock) s_gptr[i] = gptr[i];
@@ -716,7 +732,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         double4 qy = make_double4(gy[0] * ais, gy[1] * ais, gy[2] * ais, gy[3] * ais);
         s_gx[g] = qx;
         s_gy[g] = qy;
-        if (g < NP) { d.qx[gv] = qx; d.qy[gv] = qy; }
+        if (MODE != 0 && g < NP) { d.qx[gv] = qx; d.qy[gv] = qy; }
     }
     __syncthreads();
 
@@ -821,6 +837,8 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             q.w = c.GAMMA1 * (un.v[3] - 0.5 * (sq(un.v[1]) + sq(un.v[2])) / un.v[0]);
             pout[v] = q;
             if (MODE == 1) {
+                if (d.solid[v])
+                    wall_terms<VISC>(d, v, q.w, VISC ? s_mu[t] : 0.0, s_gx[t], s_gy[t]);
                 const double gra = q.x - o.x;
                 const double dif = fabs(gra);
                 red.s = sq(gra);
@@ -2088,6 +2106,16 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     TRY(dalloc(x, &d.eflux, ne));
     TRY(dalloc(x, &d.bflux, 2 * (size_t)std::max(nb, 1)));
     TRY(dalloc(x, &d.uout, (size_t)std::max(x->nff, 1)));
+    TRY(dalloc(x, &d.wall, 2 * (size_t)std::max(x->nsw, 1)));
+    TRY(hipMemset(d.wall, 0, sizeof(double4) * 2 * std::max(x->nsw, 1)) == hipSuccess ? FVM_OK : FVM_E_HIP);
+    {
+        std::vector<double2> ws(std::max(x->nsw, 1), make_double2(0.0, 0.0));
+        for (int b = 0; b < x->nsw; ++b) {
+            const double2 x1 = L.coord[L.bnode[b].x], x2 = L.coord[L.bnode[b].y];
+            ws[b] = make_double2(x2.y - x1.y, -(x2.x - x1.x));
+        }
+        TRY(upload(x, (double2 **)&d.wall_s, ws));
+    }
     TRY(dalloc(x, &d.partial, 3 * (size_t)(np / 16 + 2 + (int)L.patch_ptr.size())));
     TRY(dalloc(x, &d.hist, 6 * (size_t)kHistCap));
     TRY(dalloc(x, &x->S, 1));

@@ -1,0 +1,40 @@
+"""The autotuner's host logic: candidate launch shapes from the occupancy
+model, choice of the fastest, and the cache (timing replaced by a model)."""
+import types
+
+from fvm2d_amd import autotune
+from fvm2d_amd.solver import Inputs
+
+
+def test_candidates_fill_whole_rounds():
+    for np_ in (1_000, 4_224, 66_048, 263_168, 1_050_624):
+        c = autotune.candidates(np_)
+        assert c == sorted(set(c)) and all(24 <= ps <= 128 for ps in c)
+        if np_ >= 64 * 1024:
+            # some candidate fills its last round of 1024 resident workgroups
+            def fill(ps):
+                n = -(-np_ // ps)
+                return n / (1024 * -(-n // 1024))
+            assert max(fill(ps) for ps in c) > 0.95
+    assert 65 in autotune.candidates(66_048)           # 1017 patches: one round of 1024
+
+
+def test_tune_picks_fastest_and_caches(tmp_path):
+    geo = types.SimpleNamespace(dims=dict(np=66_048, nt=131_072))
+    inp = Inputs(iflow=1, precision=0)
+    calls = []
+
+    def timer(ps, xcd):
+        calls.append((ps, xcd))
+        return 100.0 + abs(ps - 65) + (0.5 if xcd == 0 else 0.0)
+
+    path = str(tmp_path / "tune.json")
+    c = autotune.tune(None, geo, inp, cache_path=path, device="gfx950", timer=timer)
+    assert (c.patch_size, c.xcd) == (65, 1) and c.us_per_iter == 100.0
+    assert len(calls) == 2 * len(autotune.candidates(66_048))
+    n = len(calls)
+    again = autotune.tune(None, geo, inp, cache_path=path, device="gfx950", timer=timer)
+    assert len(calls) == n and again.patch_size == 65          # from the cache
+    other = autotune.tune(None, geo, Inputs(iflow=1, precision=1), cache_path=path,
+                          device="gfx950", timer=timer)
+    assert len(calls) == 2 * n and other.patch_size == 65      # another key: timed again

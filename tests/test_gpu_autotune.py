@@ -1,0 +1,34 @@
+"""The autotuner on the GPU: it times real launch shapes, caches its choice,
+and the chosen shape computes the same states as the default one."""
+import json
+
+import numpy as np
+import pytest
+
+from fvm2d_amd import autotune
+from fvm2d_amd.solver import Inputs, Solver
+from tests.helpers import case_kwargs, grid
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_chooses_and_caches(tmp_path):
+    name = "euler_roe_lim_129x33"
+    g = grid(name)
+    inp = Inputs.from_kwargs(**case_kwargs(name))
+    with Solver(g, inp) as s:
+        geo = s.geo
+    path = str(tmp_path / "tune.json")
+    c = autotune.tune(g, geo, inp, cache_path=path, device="test", iters=16)
+    assert c.patch_size in autotune.candidates(geo.dims["np"]) and c.xcd in (0, 1)
+    assert all(t > 0 for _, _, t in c.table)
+    cache = json.load(open(path))
+    assert len(cache) == 1 and list(cache.values())[0]["patch_size"] == c.patch_size
+    states = []
+    for ps in (0, c.patch_size):
+        with Solver(g, Inputs.from_kwargs(patch_size=ps, **case_kwargs(name)), geometry=geo) as s:
+            s.tune(3, 64, c.xcd if ps else 1)
+            h, _ = s.iterate(30)
+            states.append((h, s.get_state()["prim"]))
+    np.testing.assert_array_equal(states[0][1], states[1][1])
+    np.testing.assert_array_equal(states[0][0][:, [0, 2, 3, 4, 5]], states[1][0][:, [0, 2, 3, 4, 5]])
