@@ -93,7 +93,7 @@ def tune(grid, geo, inputs, cache_path: str | None = None, device: str = "", tim
         c = cache[key]
         return Choice(c["patch_size"], c["xcd"], c["us_per_iter"], c["table"])
     timer = timer or (lambda ps, x: time_shape(grid, geo, inputs, ps, x, iters=iters))
-    per_cu = 4 if inputs.iflow == 1 else 3
+    per_cu = 3 if inputs.iflow != 1 else 4
     table = []
     for ps in candidates(geo.dims["np"], per_cu=per_cu):
         for x in (1, 0):

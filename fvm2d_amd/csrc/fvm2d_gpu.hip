@@ -44,7 +44,7 @@ namespace {
 constexpr int kBlock = 256;
 // waves per SIMD the stage kernel is compiled for (one workgroup of
 // kBlock threads puts one wave on each SIMD of a CU)
-constexpr int kPatchWavesEuler = 4, kPatchWavesVisc = 3;
+constexpr int kPatchWavesEuler = 4, kPatchWavesVisc = 3, kPatchWavesSplit = 4;
 constexpr int kHistCap = 4096;      // history rows per fvm_gpu_iterate chunk
 constexpr int kGraphIters = 8;      // iterations captured per graph
 
@@ -621,8 +621,11 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     double *s_mu = reinterpret_cast<double *>(s_coord + NLa);
     double *s_mul = s_mu + (VISC ? NLa : 0);
     double *s_nut = s_mul + (VISC ? NLa : 0);
-    double4 *s_gx = reinterpret_cast<double4 *>(s_nut + (VISC ? NLa : 0));
-    double4 *s_gy = s_gx + NG;
+    // gradients in LDS: FP64, or rounded to FP32 in the split mode, whose
+    // reconstruction reads them in FP32 (the FP64 values go to qx, qy)
+    using GT = typename std::conditional<PREC == 1, float4, double4>::type;
+    GT *s_gx = reinterpret_cast<GT *>(s_nut + (VISC ? NLa : 0));
+    GT *s_gy = s_gx + NG;
     double *s_work = reinterpret_cast<double *>(s_gy + NG);
     // work area: viscous flow stages per-triangle terms (TW doubles each);
     // inviscid flow keeps the local triangles there and recomputes each
@@ -730,8 +733,13 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         }
         double4 qx = make_double4(gx[0] * ais, gx[1] * ais, gx[2] * ais, gx[3] * ais);
         double4 qy = make_double4(gy[0] * ais, gy[1] * ais, gy[2] * ais, gy[3] * ais);
-        s_gx[g] = qx;
-        s_gy[g] = qy;
+        if (PREC == 1) {
+            s_gx[g] = GT{(float)qx.x, (float)qx.y, (float)qx.z, (float)qx.w};
+            s_gy[g] = GT{(float)qy.x, (float)qy.y, (float)qy.z, (float)qy.w};
+        } else {
+            s_gx[g] = GT{qx.x, qx.y, qx.z, qx.w};
+            s_gy[g] = GT{qy.x, qy.y, qy.z, qy.w};
+        }
         if (MODE != 0 && g < NP) { d.qx[gv] = qx; d.qy[gv] = qy; }
     }
     __syncthreads();
@@ -743,13 +751,13 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         double2 ds = ledge_ds[k];
         double2 x1 = s_coord[e.x], x2 = s_coord[e.y];
         double4 a1 = s_prim[e.x], a2 = s_prim[e.y];
-        double4 g1 = s_gx[e.x], h1 = s_gy[e.x], g2 = s_gx[e.y], h2 = s_gy[e.y];
+        const GT g1 = s_gx[e.x], h1 = s_gy[e.x], g2 = s_gx[e.y], h2 = s_gy[e.y];
         V4 p1{{a1.x, a1.y, a1.z, a1.w}}, p2{{a2.x, a2.y, a2.z, a2.w}};
-        V4 qx1{{g1.x, g1.y, g1.z, g1.w}}, qy1{{h1.x, h1.y, h1.z, h1.w}};
-        V4 qx2{{g2.x, g2.y, g2.z, g2.w}}, qy2{{h2.x, h2.y, h2.z, h2.w}};
         V4 f;
         bool ok = true;
         if (PREC == 0) {
+            V4 qx1{{g1.x, g1.y, g1.z, g1.w}}, qy1{{h1.x, h1.y, h1.z, h1.w}};
+            V4 qx2{{g2.x, g2.y, g2.z, g2.w}}, qy2{{h2.x, h2.y, h2.z, h2.w}};
             V4 pl, pr;
             recon<LIM>(c, x2.x - x1.x, x2.y - x1.y, p1, p2, qx1, qy1, qx2, qy2, pl, pr);
             if (FLUX == 1) ok = roe_flux(c, ds.x, ds.y, pl, pr, f);
@@ -761,9 +769,12 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             // (edge vector from FP64 coordinates); the flux returns to FP64
             // for the accumulation
             V4f pl, pr, ff;
+            const V4f qx1{{(float)g1.x, (float)g1.y, (float)g1.z, (float)g1.w}};
+            const V4f qy1{{(float)h1.x, (float)h1.y, (float)h1.z, (float)h1.w}};
+            const V4f qx2{{(float)g2.x, (float)g2.y, (float)g2.z, (float)g2.w}};
+            const V4f qy2{{(float)h2.x, (float)h2.y, (float)h2.z, (float)h2.w}};
             const float dxf = (float)(x2.x - x1.x), dyf = (float)(x2.y - x1.y);
-            recon<LIM, float>(c, dxf, dyf, to_f(p1), to_f(p2), to_f(qx1), to_f(qy1), to_f(qx2),
-                              to_f(qy2), pl, pr);
+            recon<LIM, float>(c, dxf, dyf, to_f(p1), to_f(p2), qx1, qy1, qx2, qy2, pl, pr);
             const float sxf = (float)ds.x, syf = (float)ds.y;
             if (FLUX == 1) ok = roe_flux<float>(c, sxf, syf, pl, pr, ff);
             else if (FLUX == 2) kfvs_flux<float>(c, sxf, syf, pl, pr, ff);
@@ -838,7 +849,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             pout[v] = q;
             if (MODE == 1) {
                 if (d.solid[v])
-                    wall_terms<VISC>(d, v, q.w, VISC ? s_mu[t] : 0.0, s_gx[t], s_gy[t]);
+                    wall_terms<VISC>(d, v, q.w, VISC ? s_mu[t] : 0.0, d.qx[v], d.qy[v]);
                 const double gra = q.x - o.x;
                 const double dif = fabs(gra);
                 red.s = sq(gra);
@@ -873,7 +884,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
 }
 
 template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
-__global__ void __launch_bounds__(kBlock, VISC ? kPatchWavesVisc : kPatchWavesEuler)
+__global__ void __launch_bounds__(kBlock, VISC ? kPatchWavesVisc : (PREC ? kPatchWavesSplit : kPatchWavesEuler))
 k_patch(Dev d, Consts c, DevState *S, PatchArgs pa, int irk, const double4 *__restrict__ pin,
         double4 *__restrict__ pout, double4 *res_out)
 {
@@ -977,7 +988,7 @@ __device__ __forceinline__ const T &uniform_at(const T *p, int i)
 }
 
 template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
-__global__ void __launch_bounds__(kBlock, VISC ? kPatchWavesVisc : kPatchWavesEuler)
+__global__ void __launch_bounds__(kBlock, VISC ? kPatchWavesVisc : (PREC ? kPatchWavesSplit : kPatchWavesEuler))
 k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work, int nwork, int xcd,
               int irk)
 {
@@ -1578,7 +1589,7 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
                   const std::vector<int> &v2e_ptr, const std::vector<int> &v2e,
                   const std::vector<int2> &enode, const std::vector<int> &v2b_ptr,
                   const std::vector<int> &v2b, const std::vector<int2> &bnode, bool visc,
-                  int lds_limit, PatchHost &ph)
+                  bool split, int lds_limit, PatchHost &ph)
 {
     const int ntri = (int)tri.size(), nedge = (int)enode.size(), nbnd = (int)bnode.size();
     std::vector<int> loc(np, -1), tloc(ntri, -1), eloc(nedge, -1), bloc(nbnd, -1);
@@ -1673,7 +1684,8 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
         if (NL > 32767 || 4 * (int)tl.size() > 32767 || 2 * (NE + 2 * NB) > 32767) return FVM_E_SIZE;
         h[PH_NGTL] = (int)ph.gtl.size() - h[PH_GTL];
         h[PH_NPSL] = (int)ph.pslot.size() - h[PH_PSL];
-        const int lds = patch_lds_bytes(NP, NL, NG, (int)tl.size(), NE, NB, h[PH_NGTL], h[PH_NPSL], visc);
+        const int lds = patch_lds_bytes(NP, NL, NG, (int)tl.size(), NE, NB, h[PH_NGTL], h[PH_NPSL], visc,
+                                        split);
         ph.lds_max = std::max(ph.lds_max, lds);
         for (int i = 0; i < kPatchHdr; ++i) ph.hdr.push_back(h[i]);
         for (int v : lv) loc[v] = -1;
@@ -1701,7 +1713,8 @@ struct Layout {
     int target = 0;                                // patch size asked of the partition
 };
 
-int build_layout(const fvm_mesh *m, int iflow, int patch_size, int lds_limit, Layout &L)
+int build_layout(const fvm_mesh *m, int iflow, int precision, int patch_size, int lds_limit,
+                 Layout &L)
 {
     const int np = m->np, nt = m->nt, ne = m->ne;
     const int nsw = std::max(0, m->nswe2 - m->nswe1 + 1), nff = std::max(0, m->nffe2 - m->nffe1 + 1);
@@ -1796,13 +1809,14 @@ int build_layout(const fvm_mesh *m, int iflow, int patch_size, int lds_limit, La
     for (;;) {
         L.ph = PatchHost();
         int rc = build_patches(np, part.patch_ptr, nadj_ptr, nadj, v2t_ptr, v2t, tri, v2e_ptr, v2e,
-                               enode, v2b_ptr, v2b, bnode, iflow != 1, lds_limit, L.ph);
+                               enode, v2b_ptr, v2b, bnode, iflow != 1, precision == 1, lds_limit,
+                               L.ph);
         if (rc == FVM_OK) break;
         if (rc != FVM_E_SIZE || target <= 16) return rc;
         target /= 2;
         part = fvm::partition_vertices(np, adj_ptr, adj, target);
         // renumbering changed: rebuild everything for the smaller patches
-        return build_layout(m, iflow, target, lds_limit, L);
+        return build_layout(m, iflow, precision, target, lds_limit, L);
     }
     L.perm = std::move(perm); L.iperm = std::move(iperm); L.patch_ptr = std::move(part.patch_ptr);
     L.nadj_ptr = std::move(nadj_ptr); L.nadj = std::move(nadj);
@@ -1954,12 +1968,13 @@ long long fvm_gpu_footprint_estimate(int np, int nt, int netot, int iflow, int p
     return b;
 }
 
-int fvm_patch_plan(const fvm_mesh *m, int iflow, int patch_size, int lds_limit, long long *st)
+int fvm_patch_plan(const fvm_mesh *m, int iflow, int precision, int patch_size, int lds_limit,
+                   long long *st)
 {
     for (int i = 0; i < 8; ++i) st[i] = 0;
     if (m->np <= 0 || m->nt <= 0 || m->ne <= 0) return FVM_E_INPUT;
     Layout L;
-    int rc = build_layout(m, iflow, patch_size, lds_limit > 0 ? lds_limit : kLdsDefault, L);
+    int rc = build_layout(m, iflow, precision, patch_size, lds_limit > 0 ? lds_limit : kLdsDefault, L);
     if (rc) return rc;
     const PatchHost &ph = L.ph;
     const int npatch = (int)L.patch_ptr.size() - 1;
@@ -2020,10 +2035,10 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
         (void)hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
     }
     x->cus = cus;
-    x->per_cu = par->iflow == 1 ? kPatchWavesEuler : kPatchWavesVisc;
+    x->per_cu = par->iflow != 1 ? kPatchWavesVisc : (par->precision == 1 ? kPatchWavesSplit : kPatchWavesEuler);
     Layout L;
     if (par->patch_size > 0) {
-        rc = build_layout(m, par->iflow, par->patch_size, lds_dev - 6 * 1024, L);
+        rc = build_layout(m, par->iflow, par->precision, par->patch_size, lds_dev - 6 * 1024, L);
     } else {
         // Whole rounds of resident workgroups, about 64 vertices a patch:
         // k rounds of cus * per_cu workgroups; fewer vertices per patch
@@ -2033,7 +2048,7 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
         for (int tries = 0;; ++tries, ++k) {
             const int ps = (int)std::min(128L, std::max(24L, (np + k * slots - 1) / (k * slots)));
             L = Layout();
-            rc = build_layout(m, par->iflow, ps, lds_dev - 6 * 1024, L);
+            rc = build_layout(m, par->iflow, par->precision, ps, lds_dev - 6 * 1024, L);
             if (rc || tries == 3 || ps == 24 || (L.ph.lds_max + 1024) * x->per_cu <= lds_cu) break;
         }
     }
@@ -2372,6 +2387,29 @@ int fvm_gpu_time_kernels(fvm_ctx *x, int n, double *ms, const char **names, int 
         if (names) names[i] = t.names[i];
     }
     return k;
+}
+
+int fvm_gpu_time_stage(fvm_ctx *x, int n, double *ms)
+{
+    *ms = 0.0;
+    if (x->variant != 3 || n <= 0) return FVM_E_INPUT;
+    if (x->hS.done) return x->hS.status;
+    // stage 1 reads the iteration-start buffer and writes buffer B only, so
+    // repeating it changes nothing the next iteration reads
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    launch_stage(self_stage(x), 1, 0);                  // warm
+    HIPCHK(hipEventRecord(e0, x->stream));
+    for (int i = 0; i < n; ++i) launch_stage(self_stage(x), 1, 0);
+    HIPCHK(hipEventRecord(e1, x->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / n;
+    return FVM_OK;
 }
 
 }  // extern "C"

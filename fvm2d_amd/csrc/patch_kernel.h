@@ -48,13 +48,14 @@ struct PatchArgs {
 
 // bytes of the LDS carve-up of one patch; the kernel and the planner both use it
 __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, int NE, int NB,
-                                               int NGTL, int NPSL, bool visc)
+                                               int NGTL, int NPSL, bool visc, bool split)
 {
     const int NLa = (NL + 1) & ~1;
     const int work = (NT * (visc ? 14 : 1) * 8) > ((NE + 2 * NB) * 32) ? NT * (visc ? 14 : 1) * 8
                                                                        : (NE + 2 * NB) * 32;
     const int index = 4 * (NG + 1) + 4 * (NP + 1) + 2 * (NGTL + NPSL);
-    return NLa * 48 + (visc ? NLa * 24 : 0) + NG * 64 + work + ((index + 15) & ~15);
+    // gradients: FP64, or FP32 in the split mode (its reconstruction reads FP32)
+    return NLa * 48 + (visc ? NLa * 24 : 0) + NG * (split ? 32 : 64) + work + ((index + 15) & ~15);
 }
 
 }  // namespace fvm

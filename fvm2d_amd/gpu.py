@@ -84,8 +84,10 @@ def lib():
     L.fvm_gpu_iterate.argtypes = [vp, ci, vp, ctypes.POINTER(ci), ctypes.POINTER(ci)]
     L.fvm_gpu_time_kernels.restype = ci
     L.fvm_gpu_time_kernels.argtypes = [vp, ci, vp, ctypes.POINTER(ctypes.c_char_p), ci]
+    L.fvm_gpu_time_stage.restype = ci
+    L.fvm_gpu_time_stage.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_double)]
     L.fvm_patch_plan.restype = ci
-    L.fvm_patch_plan.argtypes = [ctypes.POINTER(Mesh), ci, ci, ci, vp]
+    L.fvm_patch_plan.argtypes = [ctypes.POINTER(Mesh), ci, ci, ci, ci, vp]
     L.fvm_gpu_layout_info.argtypes = [vp, vp]
     L.fvm_gpu_monitor.restype = ci
     L.fvm_gpu_monitor.argtypes = [vp, vp]

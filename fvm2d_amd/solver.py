@@ -159,12 +159,13 @@ def mesh_struct(g: Geometry, keep: list) -> Mesh:
                 drmin=_p(g.drmin), wd=_p(g.wd))
 
 
-def patch_plan(geo: Geometry, iflow: int = 1, patch_size: int = 0, lds_limit: int = 0) -> dict:
+def patch_plan(geo: Geometry, iflow: int = 1, patch_size: int = 0, lds_limit: int = 0,
+               precision: int = 0) -> dict:
     """Layout of the LDS-tiled stage kernel for a mesh (host only)."""
     keep: list = []
     m = mesh_struct(geo, keep)
     st = np.zeros(8, dtype=np.int64)
-    rc = gpu.lib().fvm_patch_plan(ctypes.byref(m), iflow, patch_size, lds_limit, _p(st))
+    rc = gpu.lib().fvm_patch_plan(ctypes.byref(m), iflow, precision, patch_size, lds_limit, _p(st))
     names = ("patches", "max_patch", "max_local", "max_triangles", "max_edges", "lds_bytes",
              "halo_total", "failed_check")
     out = dict(zip(names, (int(v) for v in st)))
@@ -279,6 +280,15 @@ class Solver:
         names = (ctypes.c_char_p * 16)()
         k = self.L.fvm_gpu_time_kernels(self.h, n, _p(ms), names, 16)
         return {names[i].decode(): float(ms[i]) for i in range(min(k, 16))}
+
+    def time_stage(self, n: int = 50) -> float:
+        """Milliseconds per launch of the patch stage kernel (first RK stage,
+        n launches back to back between two HIP events)."""
+        ms = ctypes.c_double(0.0)
+        rc = self.L.fvm_gpu_time_stage(self.h, n, ctypes.byref(ms))
+        if rc != 0:
+            raise FvmError(rc, 0, "fvm_gpu_time_stage")
+        return ms.value
 
     def footprint(self) -> int:
         return int(self.L.fvm_gpu_footprint(self.h))

@@ -144,6 +144,12 @@ int fvm_gpu_iterate(fvm_ctx *ctx, int n, double *hist, int *ndone, int *err_inde
  * Returns the number of kinds. */
 int fvm_gpu_time_kernels(fvm_ctx *ctx, int n, double *ms, const char **names, int nk);
 
+/* Mean duration of the patch stage kernel over n back-to-back launches of
+ * the first RK stage on the context's stream (HIP events around the n
+ * launches, so per-launch event overhead is not counted).  Leaves the
+ * solution untouched.  No counterpart in the reference; for the roofline. */
+int fvm_gpu_time_stage(fvm_ctx *ctx, int n, double *ms);
+
 /* Tuning knobs (no counterpart in the reference; used by the autotuner):
  * variant 3 (default) = the LDS-tiled patch stage kernel, one launch per RK
  * stage; variant 1 = edge-parallel flux into a per-edge buffer + vertex
@@ -155,10 +161,10 @@ int fvm_gpu_tune(fvm_ctx *ctx, int variant, int vertices_per_block, int xcd_rema
 /* Host only (no GPU needed): plans the device layout of a mesh for the
  * LDS-tiled stage kernel and checks every index the kernels will use.
  * lds_limit: dynamic LDS bytes a workgroup may use (0 = 58 KiB; contexts use
- * the device's own limit).  stats[8]: patches, max patch vertices, max local
+ * the device's own limit); precision as in fvm_params.  stats[8]: patches, max patch vertices, max local
  * vertices (patch and its two rings), max local triangles, max local edges,
  * LDS bytes per workgroup, total halo vertices, failed check (0 = none). */
-int fvm_patch_plan(const fvm_mesh *mesh, int iflow, int patch_size, int lds_limit,
+int fvm_patch_plan(const fvm_mesh *mesh, int iflow, int precision, int patch_size, int lds_limit,
                    long long *stats);
 
 /* Bytes of device memory the context holds (admission by HBM footprint). */
