@@ -627,11 +627,13 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     GT *s_gx = reinterpret_cast<GT *>(s_nut + (VISC ? NLa : 0));
     GT *s_gy = s_gx + NG;
     double *s_work = reinterpret_cast<double *>(s_gy + NG);
-    // work area: viscous flow stages per-triangle terms (TW doubles each);
-    // inviscid flow keeps the local triangles there and recomputes each
-    // triangle's Green-Gauss term per vertex (same operands, same bits)
-    constexpr int TW = VISC ? 14 : 1;
+    // work area: first the local triangles (and, with viscosity, their
+    // areas), whose terms each vertex recomputes from LDS in its ordered sum
+    // (same operands in the same order, so the same bits as terms computed
+    // once); then the edge fluxes
+    constexpr int TW = VISC ? 2 : 1;           // doubles per local triangle
     short4 *s_ltri = reinterpret_cast<short4 *>(s_work);
+    double *s_larea = s_work + NT;
     const int WB = max(NT * TW * 8, (NE + 2 * NB) * 32);
     int *s_gptr = reinterpret_cast<int *>(reinterpret_cast<char *>(s_work) + WB);
     int *s_pptr = s_gptr + NG + 1;
@@ -658,21 +660,31 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     for (int i = t; i <= NP; i += kBlock) s_pptr[i] = pptr[i];
     for (int i = t; i < NGTL; i += kBlock) s_gtl[i] = gtl[i];
     for (int i = t; i < NPSL; i += kBlock) s_psl[i] = pslot[i];
-    if (!VISC)
-        for (int i = t; i < NT; i += kBlock) s_ltri[i] = ltri[i];
+    for (int i = t; i < NT; i += kBlock) {
+        s_ltri[i] = ltri[i];
+        if (VISC) s_larea[i] = larea[i];
+    }
     __syncthreads();
 
-    // 2. viscous flow: triangles touching G, each once
-    if (VISC) {
-        for (int k = t; k < NT; k += kBlock) {
-            short4 tr = ltri[k];
+    // 2. gradients of G (src/green_gauss.f): each vertex sums its triangles'
+    // terms in ascending triangle order; with viscosity a vertex of P also
+    // sums the triangles' Galerkin stress terms into its residual
+    // (src/green_gauss.f:132-194), evaluated here from LDS.
+    double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
+    for (int g = t; g < NG; g += kBlock) {
+        const int gv = g < NP ? v0 + g : halo[g - NP];
+        const double ais = d.ais[gv];
+        double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
+        const int k1 = s_gptr[g + 1];
+        for (int k = s_gptr[g]; k < k1; ++k) {
+            const int e = s_gtl[k];
+            const short4 tr = s_ltri[e >> 2];
             V4 p1, p2, p3;
             double bx[3], by[3], dxt[4], dyt[4];
             tri_gg(s_coord, s_prim, tr, p1, p2, p3, bx, by, dxt, dyt);
-            double *w = s_work + TW * k;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) { w[i] = dxt[i]; w[4 + i] = dyt[i]; }
-            {
+            for (int i = 0; i < 4; ++i) { gx[i] = gx[i] + dxt[i]; gy[i] = gy[i] + dyt[i]; }
+            if (VISC && g < NP) {
                 const double n2b3 = 2.0 / 3.0;
                 double e1 = p1.v[3] / p1.v[0], e2 = p2.v[3] / p2.v[0], e3 = p3.v[3] / p3.v[0];
                 double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
@@ -690,45 +702,14 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
                 double txy = mut * (dyt[1] + dxt[2]);
                 double tyy = n2b3 * mut * (2.0 * dyt[2] - dxt[1]);
                 double gpr = c.GAMMA * (mult / c.prandtl + rhot * prim5t * fv1 / c.prandtl_turb) / c.GAMMA1;
-                w[8] = txx; w[9] = txy; w[10] = tyy;
-                w[11] = txx * um + txy * vm + gpr * ext;
-                w[12] = txy * um + tyy * vm + gpr * eyt;
-                w[13] = 0.25 / larea[k];
-            }
-        }
-        __syncthreads();
-    }
-
-    // 3. gradients of G; viscous residual of P kept by P's own thread
-    double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
-    for (int g = t; g < NG; g += kBlock) {
-        const int gv = g < NP ? v0 + g : halo[g - NP];
-        const double ais = d.ais[gv];
-        double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
-        const int k1 = s_gptr[g + 1];
-        for (int k = s_gptr[g]; k < k1; ++k) {
-            const int e = s_gtl[k];
-            if (!VISC) {
-                V4 p1, p2, p3;
-                double bx[3], by[3], dxt[4], dyt[4];
-                tri_gg(s_coord, s_prim, s_ltri[e >> 2], p1, p2, p3, bx, by, dxt, dyt);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) { gx[i] = gx[i] + dxt[i]; gy[i] = gy[i] + dyt[i]; }
-                continue;
-            }
-            const double *w = s_work + TW * (e >> 2);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) { gx[i] = gx[i] + w[i]; gy[i] = gy[i] + w[4 + i]; }
-            if (g < NP) {
-                short4 tr = ltri[e >> 2];
+                double tx = txx * um + txy * vm + gpr * ext;
+                double ty = txy * um + tyy * vm + gpr * eyt;
+                double areat = 0.25 / s_larea[e >> 2];
                 const int m = e & 3;
-                double2 x1 = s_coord[tr.x], x2 = s_coord[tr.y], x3 = s_coord[tr.z];
-                double bxm = m == 0 ? x2.y - x3.y : (m == 1 ? x3.y - x1.y : x1.y - x2.y);
-                double bym = m == 0 ? x3.x - x2.x : (m == 1 ? x1.x - x3.x : x2.x - x1.x);
-                double areat = w[13];
-                rv1 = rv1 + (w[8] * bxm + w[9] * bym) * areat;
-                rv2 = rv2 + (w[9] * bxm + w[10] * bym) * areat;
-                rv3 = rv3 + (w[11] * bxm + w[12] * bym) * areat;
+                const double bxm = bx[m], bym = by[m];
+                rv1 = rv1 + (txx * bxm + txy * bym) * areat;
+                rv2 = rv2 + (txy * bxm + tyy * bym) * areat;
+                rv3 = rv3 + (tx * bxm + ty * bym) * areat;
             }
         }
         double4 qx = make_double4(gx[0] * ais, gx[1] * ais, gx[2] * ais, gx[3] * ais);
@@ -744,7 +725,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     }
     __syncthreads();
 
-    // 4. fluxes of the edges and boundary edges touching P
+    // 3. fluxes of the edges and boundary edges touching P
     int bad = INT_MAX;
     for (int k = t; k < NE; k += kBlock) {
         short2 e = ledge[k];
@@ -807,7 +788,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     }
     __syncthreads();
 
-    // 5. residual of P in the reference's order, then the RK stage
+    // 4. residual of P in the reference's order, then the RK stage
     Red red{0.0, 0.0, 0};
     if (t < NP) {
         const int v = v0 + t;

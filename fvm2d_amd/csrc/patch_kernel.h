@@ -3,11 +3,13 @@
 //
 // A workgroup owns a patch P of consecutive vertices.  It loads the stage
 // state of P, its 1-ring H1 and its 2-ring H2 into LDS, then
-//   1. viscous flow: evaluates every triangle touching G = P + H1 once
-//      (Green-Gauss terms and the Galerkin stresses);
-//   2. sums, for each vertex of G, its triangles in ascending triangle
-//      order -> gradients of G (src/green_gauss.f); inviscid flow evaluates
-//      each triangle's Green-Gauss term in this sum, from LDS;
+//   1. (with the gather) copies the local triangles into LDS;
+//   2. sums, for each vertex of G = P + H1, its triangles' Green-Gauss
+//      terms in ascending triangle order -> gradients of G
+//      (src/green_gauss.f); with viscosity each vertex of P also sums the
+//      triangles' Galerkin stress terms; every term is evaluated in the sum
+//      from LDS (same operands, same order: the same bits as terms
+//      evaluated once per triangle);
 //   3. evaluates every edge touching P once (MUSCL + Riemann flux) and every
 //      solid/farfield edge touching P (src/residu.f:37-88);
 //   4. sums, for each vertex of P, its edge and boundary terms in the
@@ -51,8 +53,8 @@ __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, i
                                                int NGTL, int NPSL, bool visc, bool split)
 {
     const int NLa = (NL + 1) & ~1;
-    const int work = (NT * (visc ? 14 : 1) * 8) > ((NE + 2 * NB) * 32) ? NT * (visc ? 14 : 1) * 8
-                                                                       : (NE + 2 * NB) * 32;
+    const int work = (NT * (visc ? 2 : 1) * 8) > ((NE + 2 * NB) * 32) ? NT * (visc ? 2 : 1) * 8
+                                                                      : (NE + 2 * NB) * 32;
     const int index = 4 * (NG + 1) + 4 * (NP + 1) + 2 * (NGTL + NPSL);
     // gradients: FP64, or FP32 in the split mode (its reconstruction reads FP32)
     return NLa * 48 + (visc ? NLa * 24 : 0) + NG * (split ? 32 : 64) + work + ((index + 15) & ~15);
