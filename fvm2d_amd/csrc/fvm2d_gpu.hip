@@ -295,25 +295,6 @@ __device__ Red block_reduce(Red x)
     return r;
 }
 
-template <int NT>
-__device__ double2 block_sum2(double2 x)
-{
-    __shared__ double2 s2[NT / 64];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        x.x = x.x + __shfl_xor(x.x, o);
-        x.y = x.y + __shfl_xor(x.y, o);
-    }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) s2[w] = x;
-    __syncthreads();
-    double2 r = s2[0];
-#pragma unroll
-    for (int k = 1; k < NT / 64; ++k) { r.x = r.x + s2[k].x; r.y = r.y + s2[k].y; }
-    __syncthreads();
-    return r;
-}
-
 // What src/clcd.f:26-34 reads at a solid point, left by the last stage for
 // every solid edge the point ends: the new pressure and, with viscosity, the
 // stress of the last residual's gradients.  k_final then adds the edges in
@@ -491,56 +472,104 @@ __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int
 // viscosity the stress terms are zero and are left out.
 constexpr int kFinal = 1024;
 
+// force of solid edge b on (xf, yf) from the wall terms its two end points
+// left (src/clcd.f:18-43)
+template <bool VISC>
+__device__ __forceinline__ void wall_force(double4 w1, double4 w2, double2 sn, double &xf, double &yf)
+{
+    const double sx = sn.x, sy = sn.y;
+    double p = 0.5 * (w1.x + w2.x);
+    if (VISC) {
+        double txx = 0.5 * (w1.y + w2.y), txy = 0.5 * (w1.z + w2.z), tyy = 0.5 * (w1.w + w2.w);
+        xf = xf + p * sx - txx * sx - txy * sy;
+        yf = yf + p * sy - txy * sx - tyy * sy;
+    } else {
+        xf = xf + p * sx;
+        yf = yf + p * sy;
+    }
+}
+
+// (sum, max, argmax) and a force pair over the workgroup in one pass of
+// barriers; each component follows the butterfly and the ordered sum of
+// wavefront totals of block_reduce
+template <int NT>
+__device__ void block_reduce_final(Red &x, double2 &f)
+{
+    __shared__ double ss[NT / 64], sm[NT / 64];
+    __shared__ int si[NT / 64];
+    __shared__ double2 s2[NT / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        x = red_join(x, shfl_red(x, o));
+        f.x = f.x + __shfl_xor(f.x, o);
+        f.y = f.y + __shfl_xor(f.y, o);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[w] = x.s; sm[w] = x.m; si[w] = x.i; s2[w] = f; }
+    __syncthreads();
+    Red r{ss[0], sm[0], si[0]};
+    double2 g = s2[0];
+#pragma unroll
+    for (int k = 1; k < NT / 64; ++k) {
+        r = red_join(r, Red{ss[k], sm[k], si[k]});
+        g.x = g.x + s2[k].x; g.y = g.y + s2[k].y;
+    }
+    x = r;
+    f = g;
+}
+
 template <bool VISC>
 __device__ __forceinline__ void final_body(const Dev &d, const Consts &c, DevState *S)
 {
-    if (S->done) return;
-    const int err = S->err_code;
+    // Every load the monitors need (the solve's state words, the first
+    // partial and the first solid edge of each thread) is issued before any
+    // is used, so the workgroup waits for memory once; only the writes
+    // depend on whether the solve is still running.
+    const int done = S->done, err = S->err_code;
+    const int iter = S->iter, nhist = S->nhist;
+    const double fres1 = S->fres1;
     const int t = threadIdx.x;
-    if (err) {
-        if (t == 0) { S->status = err; S->done = 1; }
-        return;
-    }
     Red acc{0.0, 0.0, 0};
-#pragma unroll 2
-    for (int k = t; k < d.nparts; k += kFinal)
-        acc = red_join(acc, Red{d.partial[3 * k], d.partial[3 * k + 1], (int)d.partial[3 * k + 2]});
-    // forces on the solid edges, src/clcd.f:18-43
     double xf = 0.0, yf = 0.0;
-#pragma unroll 1
-    for (int b = t; b < d.nsw; b += kFinal) {
-        const double4 w1 = d.wall[2 * b], w2 = d.wall[2 * b + 1];
-        const double2 sn = d.wall_s[b];
-        const double sx = sn.x, sy = sn.y;
-        double p = 0.5 * (w1.x + w2.x);
-        if (VISC) {
-            double txx = 0.5 * (w1.y + w2.y), txy = 0.5 * (w1.z + w2.z), tyy = 0.5 * (w1.w + w2.w);
-            xf = xf + p * sx - txx * sx - txy * sy;
-            yf = yf + p * sy - txy * sx - tyy * sy;
-        } else {
-            xf = xf + p * sx;
-            yf = yf + p * sy;
-        }
+    {
+        const bool hp = t < d.nparts, hw = t < d.nsw;
+        double ps = 0.0, pm = 0.0, pi = 0.0;
+        double4 w1 = make_double4(0.0, 0.0, 0.0, 0.0), w2 = w1;
+        double2 sn = make_double2(0.0, 0.0);
+        if (hp) { ps = d.partial[3 * t]; pm = d.partial[3 * t + 1]; pi = d.partial[3 * t + 2]; }
+        if (hw) { w1 = d.wall[2 * t]; w2 = d.wall[2 * t + 1]; sn = d.wall_s[t]; }
+        if (hp) acc = red_join(acc, Red{ps, pm, (int)pi});
+        if (hw) wall_force<VISC>(w1, w2, sn, xf, yf);
     }
-    const Red tot = block_reduce<kFinal>(acc);
-    const double2 f = block_sum2<kFinal>(make_double2(xf, yf));
+#pragma unroll 1
+    for (int k = t + kFinal; k < d.nparts; k += kFinal)
+        acc = red_join(acc, Red{d.partial[3 * k], d.partial[3 * k + 1], (int)d.partial[3 * k + 2]});
+#pragma unroll 1
+    for (int b = t + kFinal; b < d.nsw; b += kFinal)
+        wall_force<VISC>(d.wall[2 * b], d.wall[2 * b + 1], d.wall_s[b], xf, yf);
+    double2 f = make_double2(xf, yf);
+    block_reduce_final<kFinal>(acc, f);
+    if (done || t != 0) return;
+    if (err) { S->status = err; S->done = 1; return; }
+    const Red tot = acc;
     double qdyn = 0.5 * c.r_inf * sq(c.q_inf);
     double cl = (-c.sin_aoa * f.x + c.cos_aoa * f.y) / qdyn;
     double cd = (c.cos_aoa * f.x + c.sin_aoa * f.y) / qdyn;
-    if (t != 0) return;
     double fres = sqrt(tot.s / c.np);
-    if (S->iter == c.iterlast) {
-        S->fres1 = fres;
+    double f1 = fres1;
+    if (iter == c.iterlast) {
+        S->fres1 = f1 = fres;
         if (fres == 0.0) { S->status = FVM_E_ZERO_RES1; S->err_code = FVM_E_ZERO_RES1; S->done = 1; return; }
     }
-    fres = fres / S->fres1;
+    fres = fres / f1;
     S->fres = fres; S->fresi = tot.m; S->iresi = tot.i; S->cl = cl; S->cd = cd;
-    if (S->nhist < kHistCap) {
-        double *h = d.hist + 6 * S->nhist;
-        h[0] = (double)S->iter; h[1] = fres; h[2] = (double)tot.i; h[3] = tot.m; h[4] = cl; h[5] = cd;
-        S->nhist++;
+    if (nhist < kHistCap) {
+        double *h = d.hist + 6 * nhist;
+        h[0] = (double)iter; h[1] = fres; h[2] = (double)tot.i; h[3] = tot.m; h[4] = cl; h[5] = cd;
+        S->nhist = nhist + 1;
     }
-    int it = ++S->iter;
+    int it = iter + 1;
+    S->iter = it;
     if (!(fres > c.minres)) { S->done = 1; S->status = FVM_CONVERGED; }
     else if (it >= c.iterlast + c.maxiter) { S->done = 1; S->status = FVM_ITER_LIMIT; }
     else if (c.saveinterval > 0 && it % c.saveinterval == 0 && it >= c.maxiter) {
@@ -571,6 +600,17 @@ __device__ __forceinline__ int logical_block(int nb, int xcd)
 
 // Green-Gauss term of one triangle, src/green_gauss.f:97-127: for each
 // variable, sum over the corners of q_k (dy, -dx) of the opposite side.
+// Descriptors, patch headers and work lists are read through the constant address space:
+// they do not change during a launch, so the compiler treats them like
+// kernel arguments (scalar loads, reloaded where needed instead of held in
+// vector registers across the kernel's stores).
+template <class T>
+__device__ __forceinline__ const T &uniform_at(const T *p, int i)
+{
+    using CT = __attribute__((address_space(4))) const T;
+    return *(const T *)((const CT *)p + i);
+}
+
 __device__ __forceinline__ void tri_gg(const double2 *s_coord, const double4 *s_prim, short4 tr,
                                        V4 &p1, V4 &p2, V4 &p3, double bx[3], double by[3],
                                        double dxt[4], double dyt[4])
@@ -593,26 +633,31 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
                                            const double4 *__restrict__ pin,
                                            double4 *__restrict__ pout, double4 *res_out)
 {
-    if (S->done || S->err_code) return;
+    // The solve's status words are loaded with the patch header and tested
+    // after the gather, which writes only LDS: their latency hides under it.
+    // The header is read through the constant address space (scalar loads).
+    const int stop = S->done | S->err_code;
+    const int hb = kPatchHdr * b;
+    auto hdr = [&](int k) { return uniform_at(pa.hdr, hb + k); };
+    const int v0 = hdr(PH_V0), NP = hdr(PH_NP), NL = hdr(PH_NL), NG = hdr(PH_NG);
+    const int NT = hdr(PH_NTRI), NE = hdr(PH_NEDGE), NB = hdr(PH_NBND);
+    const int NGTL = hdr(PH_NGTL), NPSL = hdr(PH_NPSL);
     extern __shared__ double4 smem4[];
     double *smem = reinterpret_cast<double *>(smem4);
-    const int *h = pa.hdr + kPatchHdr * b;
-    const int v0 = h[PH_V0], NP = h[PH_NP], NL = h[PH_NL], NG = h[PH_NG];
     const int NLa = (NL + 1) & ~1;               // keeps every LDS array 16-byte aligned
-    const int NT = h[PH_NTRI], NE = h[PH_NEDGE], NB = h[PH_NBND];
-    const int NGTL = h[PH_NGTL], NPSL = h[PH_NPSL];
-    const int *halo = pa.halo + h[PH_HALO];
-    const short4 *ltri = pa.ltri + h[PH_TRI];
-    const double *larea = pa.larea + h[PH_TRI];
-    const int *gptr = pa.gptr + h[PH_GPTR];
-    const short *gtl = pa.gtl + h[PH_GTL];
-    const short2 *ledge = pa.ledge + h[PH_EDGE];
-    const int *ledge_id = pa.ledge_id + h[PH_EDGE];
-    const double2 *ledge_ds = pa.ledge_ds + h[PH_EDGE];
-    const short2 *lbnd = pa.lbnd + h[PH_BND];
-    const int *lbnd_id = pa.lbnd_id + h[PH_BND];
-    const int *pptr = pa.pptr + h[PH_PPTR];
-    const short *pslot = pa.pslot + h[PH_PSL];
+    const int NGa = (NG + 1) & ~1;
+    const int *halo = pa.halo + hdr(PH_HALO);
+    const short4 *ltri = pa.ltri + hdr(PH_TRI);
+    const double *larea = pa.larea + hdr(PH_TRI);
+    const int *gptr = pa.gptr + hdr(PH_GPTR);
+    const short *gtl = pa.gtl + hdr(PH_GTL);
+    const short2 *ledge = pa.ledge + hdr(PH_EDGE);
+    const int *ledge_id = pa.ledge_id + hdr(PH_EDGE);
+    const double2 *ledge_ds = pa.ledge_ds + hdr(PH_EDGE);
+    const short2 *lbnd = pa.lbnd + hdr(PH_BND);
+    const int *lbnd_id = pa.lbnd_id + hdr(PH_BND);
+    const int *pptr = pa.pptr + hdr(PH_PPTR);
+    const short *pslot = pa.pslot + hdr(PH_PSL);
     const int t = threadIdx.x;
 
     // LDS carve-up for this patch
@@ -626,7 +671,10 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     using GT = typename std::conditional<PREC == 1, float4, double4>::type;
     GT *s_gx = reinterpret_cast<GT *>(s_nut + (VISC ? NLa : 0));
     GT *s_gy = s_gx + NG;
-    double *s_work = reinterpret_cast<double *>(s_gy + NG);
+    // the gradients' area factor ais of G, gathered with the state so that
+    // the gradient phase reads no global memory
+    double *s_ais = reinterpret_cast<double *>(s_gy + NG);
+    double *s_work = s_ais + NGa;
     // work area: first the local triangles (and, with viscosity, their
     // areas), whose terms each vertex recomputes from LDS in its ordered sum
     // (same operands in the same order, so the same bits as terms computed
@@ -646,14 +694,12 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         double4 p = pin[gv];
         s_prim[i] = p;
         s_coord[i] = d.coord[gv];
+        if (i < NG) s_ais[i] = d.ais[gv];
         if (VISC) {
             double ml = sutherland_viscosity(c, p.x, p.w);
             double nu = d.nut[gv];
             double m = TURB ? ml + mu_turb(c, ml, p.x, nu) : ml;
             s_mul[i] = ml; s_mu[i] = m; s_nut[i] = nu;
-            // mu, mul, qx, qy of the last residual of an iteration are what
-            // the monitors, the SA step and the caller read
-            if (MODE != 0 && i < NP) { d.mul[gv] = ml; d.mu[gv] = m; }
         }
     }
     for (int i = t; i <= NG; i += kBlock) s_gptr[i] = gptr[i];
@@ -665,6 +711,31 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         if (VISC) s_larea[i] = larea[i];
     }
     __syncthreads();
+    if (stop) return;
+
+    // Solid and farfield fluxes need no gradient: they are evaluated during
+    // the gradient phase by threads from the top of the workgroup (which the
+    // gradients, one vertex of G per thread, leave idle), unless their LDS
+    // slots, after the NE edge terms, still hold local triangles then.
+    auto bnd_flux = [&](const int kb) {
+        short2 e = lbnd[kb];
+        const int gb = lbnd_id[kb];
+        double2 x1 = s_coord[e.x], x2 = s_coord[e.y];
+        V4 r1, r2;
+        if (gb < d.nsw) {
+            solid_flux(x1, x2, s_prim[e.x].w, s_prim[e.y].w, r1, r2);
+        } else {
+            double4 a1 = s_prim[e.x], a2 = s_prim[e.y];
+            V4 p1{{a1.x, a1.y, a1.z, a1.w}}, p2{{a2.x, a2.y, a2.z, a2.w}};
+            farfield_flux(c, d.uout[gb - d.nsw], x1, x2, p1, p2, r1, r2);
+        }
+        double4 *fw = reinterpret_cast<double4 *>(s_work) + NE + 2 * kb;
+        fw[0] = make_double4(r1.v[0], r1.v[1], r1.v[2], r1.v[3]);
+        fw[1] = make_double4(r2.v[0], r2.v[1], r2.v[2], r2.v[3]);
+    };
+    const bool bnd_early = NE * 32 >= NT * TW * 8;
+    if (bnd_early)
+        for (int kb = kBlock - 1 - t; kb < NB; kb += kBlock) bnd_flux(kb);
 
     // 2. gradients of G (src/green_gauss.f): each vertex sums its triangles'
     // terms in ascending triangle order; with viscosity a vertex of P also
@@ -672,16 +743,10 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     // (src/green_gauss.f:132-194), evaluated here from LDS.
     double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
     for (int g = t; g < NG; g += kBlock) {
-        const int gv = g < NP ? v0 + g : halo[g - NP];
-        const double ais = d.ais[gv];
+        const double ais = s_ais[g];
         double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
-        const int k1 = s_gptr[g + 1];
-        for (int k = s_gptr[g]; k < k1; ++k) {
-            const int e = s_gtl[k];
-            const short4 tr = s_ltri[e >> 2];
-            V4 p1, p2, p3;
-            double bx[3], by[3], dxt[4], dyt[4];
-            tri_gg(s_coord, s_prim, tr, p1, p2, p3, bx, by, dxt, dyt);
+        auto add_tri = [&](const int e, const short4 tr, const V4 &p1, const V4 &p2, const V4 &p3,
+                           const double *bx, const double *by, const double *dxt, const double *dyt) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) { gx[i] = gx[i] + dxt[i]; gy[i] = gy[i] + dyt[i]; }
             if (VISC && g < NP) {
@@ -711,6 +776,15 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
                 rv2 = rv2 + (txy * bxm + tyy * bym) * areat;
                 rv3 = rv3 + (tx * bxm + ty * bym) * areat;
             }
+        };
+        const int k1 = s_gptr[g + 1];
+        for (int k = s_gptr[g]; k < k1; ++k) {
+            const int e = s_gtl[k];
+            const short4 tr = s_ltri[e >> 2];
+            V4 p1, p2, p3;
+            double bx[3], by[3], dxt[4], dyt[4];
+            tri_gg(s_coord, s_prim, tr, p1, p2, p3, bx, by, dxt, dyt);
+            add_tri(e, tr, p1, p2, p3, bx, by, dxt, dyt);
         }
         double4 qx = make_double4(gx[0] * ais, gx[1] * ais, gx[2] * ais, gx[3] * ais);
         double4 qy = make_double4(gy[0] * ais, gy[1] * ais, gy[2] * ais, gy[3] * ais);
@@ -721,7 +795,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             s_gx[g] = GT{qx.x, qx.y, qx.z, qx.w};
             s_gy[g] = GT{qy.x, qy.y, qy.z, qy.w};
         }
-        if (MODE != 0 && g < NP) { d.qx[gv] = qx; d.qy[gv] = qy; }
+        if (MODE != 0 && g < NP) { d.qx[v0 + g] = qx; d.qy[v0 + g] = qy; }
     }
     __syncthreads();
 
@@ -766,22 +840,8 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         if (!ok) bad = min(bad, ledge_id[k] + 1);
         reinterpret_cast<double4 *>(s_work)[k] = make_double4(f.v[0], f.v[1], f.v[2], f.v[3]);
     }
-    for (int k = t; k < NB; k += kBlock) {
-        short2 e = lbnd[k];
-        const int gb = lbnd_id[k];
-        double2 x1 = s_coord[e.x], x2 = s_coord[e.y];
-        V4 r1, r2;
-        if (gb < d.nsw) {
-            solid_flux(x1, x2, s_prim[e.x].w, s_prim[e.y].w, r1, r2);
-        } else {
-            double4 a1 = s_prim[e.x], a2 = s_prim[e.y];
-            V4 p1{{a1.x, a1.y, a1.z, a1.w}}, p2{{a2.x, a2.y, a2.z, a2.w}};
-            farfield_flux(c, d.uout[gb - d.nsw], x1, x2, p1, p2, r1, r2);
-        }
-        double4 *fw = reinterpret_cast<double4 *>(s_work) + NE + 2 * k;
-        fw[0] = make_double4(r1.v[0], r1.v[1], r1.v[2], r1.v[3]);
-        fw[1] = make_double4(r2.v[0], r2.v[1], r2.v[2], r2.v[3]);
-    }
+    if (!bnd_early)
+        for (int kb = t; kb < NB; kb += kBlock) bnd_flux(kb);
     if (bad != INT_MAX) {
         atomicMin(&S->err_index, bad);
         S->err_code = FVM_E_NEG_SOUND;
@@ -801,15 +861,28 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         r.v[1] = VISC ? rv1 : 0.0; r.v[2] = VISC ? rv2 : 0.0; r.v[3] = VISC ? rv3 : 0.0;
         const int k1 = s_pptr[t + 1];
         const double4 *fl = reinterpret_cast<const double4 *>(s_work);
-        for (int k = s_pptr[t]; k < k1; ++k) {
-            const int q = s_psl[k];
-            double4 f = fl[q >> 1];
+        auto apply = [&](const int q, const double4 f) {
             if (q & 1) {
                 r.v[0] = r.v[0] - f.x; r.v[1] = r.v[1] - f.y; r.v[2] = r.v[2] - f.z; r.v[3] = r.v[3] - f.w;
             } else {
                 r.v[0] = r.v[0] + f.x; r.v[1] = r.v[1] + f.y; r.v[2] = r.v[2] + f.z; r.v[3] = r.v[3] + f.w;
             }
+        };
+        // two terms a step, read before either is added, added in order
+        int k = s_pptr[t];
+        for (; k + 1 < k1; k += 2) {
+            const int qa = s_psl[k], qb = s_psl[k + 1];
+            const double4 fa = fl[qa >> 1], fb = fl[qb >> 1];
+            apply(qa, fa);
+            apply(qb, fb);
         }
+        if (k < k1) {
+            const int q = s_psl[k];
+            apply(q, fl[q >> 1]);
+        }
+        // mu, mul, qx, qy of the last residual of an iteration are what the
+        // monitors, the SA step and the caller read
+        if (VISC && MODE != 0) { d.mul[v] = s_mul[t]; d.mu[v] = s_mu[t]; }
         if (MODE == 2) {
             st4(res_out, v, r);
         } else {
@@ -957,17 +1030,6 @@ struct BatchMember {
     double4 *pbuf[3];      // stage buffers O, B, C
 };
 
-// Descriptors and work lists are read through the constant address space:
-// they do not change during a launch, so the compiler treats them like
-// kernel arguments (scalar loads, reloaded where needed instead of held in
-// vector registers across the kernel's stores).
-template <class T>
-__device__ __forceinline__ const T &uniform_at(const T *p, int i)
-{
-    using CT = __attribute__((address_space(4))) const T;
-    return *(const T *)((const CT *)p + i);
-}
-
 template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
 __global__ void __launch_bounds__(kBlock, VISC ? kPatchWavesVisc : (PREC ? kPatchWavesSplit : kPatchWavesEuler))
 k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work, int nwork, int xcd,
@@ -1064,6 +1126,7 @@ struct fvm_ctx {
     std::vector<int> perm;            // device vertex -> caller's vertex (0-based)
     BatchMember *self_mem = nullptr;  // this solve as a batch of one
     int2 *self_work = nullptr;
+    std::vector<int> patch_cost;      // per patch: edges + local vertices (work estimate)
     struct fvm_batch *batch = nullptr;   // the batch it belongs to, if any
 };
 
@@ -1225,6 +1288,52 @@ void launch_update(fvm_ctx *x)
     else hipLaunchKernelGGL((k_update<IRK, true, LAST, true>), g, b, 0, x->stream, x->d, x->c, x->S);
 }
 
+
+// Order of a stage launch's work list (a performance choice only: every
+// result is the same for every order).  The XCD-aware mapping hands the
+// list to the 8 XCDs as contiguous ranges.  When a range fits one round of
+// resident workgroups, the k-th workgroup an XCD receives runs on CU
+// k mod C of it (C CUs per XCD; the dispatcher deals a round to the CUs in
+// turn), so the items are dealt heaviest first to the least loaded CU and
+// placed at that CU's positions j, j + C, ...: each CU gets a mix of heavy
+// and light patches.  With more rounds, the range is sorted heaviest first,
+// so that the last round holds the light ones.
+void order_work(std::vector<int2> &work, const std::vector<int> &cost, int cus, int per_cu)
+{
+    const int n = (int)work.size();
+    const int C = std::max(1, cus / 8);
+    const int q = n >> 3, r = n & 7;
+    std::vector<int2> out(work.size());
+    for (int x = 0; x < 8; ++x) {
+        const int lo = x * q + std::min(x, r), m = q + (x < r ? 1 : 0);
+        if (m == 0) continue;
+        std::vector<int> idx(m);
+        std::iota(idx.begin(), idx.end(), lo);
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        if (m > C * per_cu) {
+            for (int i = 0; i < m; ++i) out[lo + i] = work[idx[i]];
+            continue;
+        }
+        // CU j owns local positions j, j + C, ... below m
+        std::vector<long long> load(C, 0);
+        std::vector<int> fill(C, 0), cap(C);
+        for (int j = 0; j < C; ++j) cap[j] = j < m ? (m - 1 - j) / C + 1 : 0;
+        for (int i = 0; i < m; ++i) {
+            int best = -1;
+            for (int j = 0; j < C; ++j)
+                if (fill[j] < cap[j] && (best < 0 || load[j] < load[best])) best = j;
+            out[lo + best + C * fill[best]] = work[idx[i]];
+            load[best] += cost[idx[i]];
+            fill[best]++;
+        }
+    }
+    work.swap(out);
+}
+
+int effective_per_cu(const fvm_ctx *x, int lds_bytes)
+{
+    return std::max(1, std::min(x->per_cu, x->lds_per_cu / (lds_bytes + 1024)));
+}
 
 // Stage kernels read the stage state `pin` and write `pout`; the buffers
 // rotate O -> B -> C -> O over the three stages, O being the state at the
@@ -2086,9 +2195,12 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     x->target = L.target;
     {
         long long halo = 0; int maxp = 0;
+        x->patch_cost.resize(x->pa.npatch);
         for (int k = 0; k < x->pa.npatch; ++k) {
-            halo += L.ph.hdr[kPatchHdr * k + PH_NL] - L.ph.hdr[kPatchHdr * k + PH_NP];
-            maxp = std::max(maxp, L.ph.hdr[kPatchHdr * k + PH_NP]);
+            const int *hk = &L.ph.hdr[kPatchHdr * k];
+            halo += hk[PH_NL] - hk[PH_NP];
+            maxp = std::max(maxp, hk[PH_NP]);
+            x->patch_cost[k] = hk[PH_NEDGE] + hk[PH_NL];
         }
         x->halo_total = halo; x->max_patch = maxp;
     }
@@ -2138,6 +2250,7 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     {
         std::vector<int2> w(x->pa.npatch);
         for (int k = 0; k < x->pa.npatch; ++k) w[k] = make_int2(0, k);
+        order_work(w, x->patch_cost, x->cus, effective_per_cu(x, x->pa.lds_bytes));
         TRY(upload(x, (int2 **)&x->self_work, w));
     }
 #undef TRY
@@ -2430,15 +2543,20 @@ int batch_rebuild(fvm_batch *b)
     std::vector<BatchMember> m;
     std::vector<int2> work, vwork;
     b->lds = 0; b->max_nff = 0; b->any_vortex = false;
+    std::vector<int> cost;
     for (int i = 0; i < (int)b->mem.size(); ++i) {
         const fvm_ctx *x = b->mem[i];
         m.push_back(member_of(x));
-        for (int k = 0; k < x->pa.npatch; ++k) work.push_back(make_int2(i, k));
+        for (int k = 0; k < x->pa.npatch; ++k) {
+            work.push_back(make_int2(i, k));
+            cost.push_back(x->patch_cost[k]);
+        }
         for (int k = 0; k < (int)nblk(x->np); ++k) vwork.push_back(make_int2(i, k));
         b->lds = std::max(b->lds, x->pa.lds_bytes);
         b->max_nff = std::max(b->max_nff, x->nff);
         if (x->c.vortex == 1 && x->c.mach_inf < 1.0) b->any_vortex = true;
     }
+    if (!b->mem.empty()) order_work(work, cost, b->mem[0]->cus, effective_per_cu(b->mem[0], b->lds));
     int rc = batch_upload(&b->d_mem, m);
     if (!rc) rc = batch_upload(&b->d_work, work);
     if (!rc) rc = batch_upload(&b->d_vwork, vwork);

@@ -329,46 +329,40 @@ __device__ inline void farfield_flux(const Consts &c, const double4 uo, double2 
     lp[0] = fmax(lam[0], 0.0); lp[1] = lp[0]; lp[2] = fmax(lam[2], 0.0); lp[3] = fmax(lam[3], 0.0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) lm[k] = lam[k] - lp[k];
-    // T (right eigenvectors) row by row and S (its inverse) column by column,
-    // built when needed: jac(i,j) = sum_k (l_k T(i,k)) S(k,j) in the
-    // reference's order, with a few live registers instead of two 4x4
-    // matrices and their products.
+    // S (the inverse of the right eigenvectors T) is evaluated once, entry by
+    // entry with the reference's operations; T row by row.  jac(i,j) =
+    // sum_k (l_k T(i,k)) S(k,j) in the reference's order.  Both loops are
+    // unrolled, so the divisions are independent of one another.
     const double t1 = 0.5 * ra / aa, m2 = qa2 / aa2;
     const double s1 = 0.5 * G1 * m2 * aa / ra, s2 = una / ra, s3 = G1 * ua / aa, s4 = G1 * va / aa,
                  s5 = G1 / (ra * aa);
-#pragma unroll 1
+    double S[4][4];                         // S[j][k] = S(k, j)
+    S[0][0] = 1.0 - 0.5 * G1 * m2; S[0][1] = vna / ra;  S[0][2] = s1 - s2;        S[0][3] = s1 + s2;
+    S[1][0] = s3 / aa;             S[1][1] = st / ra;   S[1][2] = (ct - s3) / ra; S[1][3] = -(ct + s3) / ra;
+    S[2][0] = s4 / aa;             S[2][1] = -ct / ra;  S[2][2] = (st - s4) / ra; S[2][3] = -(st + s4) / ra;
+    S[3][0] = -G1 / aa2;           S[3][1] = 0.0;       S[3][2] = s5;             S[3][3] = s5;
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
         double T0, T1, T2, T3;
         if (i == 0)      { T0 = 1.0;       T1 = 0.0;        T2 = t1;                 T3 = t1; }
         else if (i == 1) { T0 = ua;        T1 = ra * st;    T2 = t1 * (ua + aa * ct); T3 = t1 * (ua - aa * ct); }
         else if (i == 2) { T0 = va;        T1 = -ra * ct;   T2 = t1 * (va + aa * st); T3 = t1 * (va - aa * st); }
         else             { T0 = 0.5 * qa2; T1 = -ra * vna;  T2 = t1 * (ha + aa * una); T3 = t1 * (ha - aa * una); }
+        const double pa0 = lp[0] * T0, pa1 = lp[1] * T1, pa2 = lp[2] * T2, pa3 = lp[3] * T3;
+        const double pb0 = lm[0] * T0, pb1 = lm[1] * T1, pb2 = lm[2] * T2, pb3 = lm[3] * T3;
         double f1 = 0.0, f2 = 0.0;
-#pragma unroll 1
+#pragma unroll
         for (int j = 0; j < 4; ++j) {
-            double S0, S1, S2, S3;
-            if (j == 0)      { S0 = 1.0 - 0.5 * G1 * m2; S1 = vna / ra;  S2 = s1 - s2;        S3 = s1 + s2; }
-            else if (j == 1) { S0 = s3 / aa;             S1 = st / ra;   S2 = (ct - s3) / ra; S3 = -(ct + s3) / ra; }
-            else if (j == 2) { S0 = s4 / aa;             S1 = -ct / ra;  S2 = (st - s4) / ra; S3 = -(st + s4) / ra; }
-            else             { S0 = -G1 / aa2;           S1 = 0.0;       S2 = s5;             S3 = s5; }
             double a = 0.0, b = 0.0;
-            a = a + (lp[0] * T0) * S0; b = b + (lm[0] * T0) * S0;
-            a = a + (lp[1] * T1) * S1; b = b + (lm[1] * T1) * S1;
-            a = a + (lp[2] * T2) * S2; b = b + (lm[2] * T2) * S2;
-            a = a + (lp[3] * T3) * S3; b = b + (lm[3] * T3) * S3;
-            // select rather than index: a runtime index would put the
-            // vectors in scratch memory
-            const double u1 = j == 0 ? U1.v[0] : (j == 1 ? U1.v[1] : (j == 2 ? U1.v[2] : U1.v[3]));
-            const double u2 = j == 0 ? U2.v[0] : (j == 1 ? U2.v[1] : (j == 2 ? U2.v[2] : U2.v[3]));
-            const double uo = j == 0 ? Uout[0] : (j == 1 ? Uout[1] : (j == 2 ? Uout[2] : Uout[3]));
-            f1 = f1 + a * u1 + b * uo;
-            f2 = f2 + a * u2 + b * uo;
+            a = a + pa0 * S[j][0]; b = b + pb0 * S[j][0];
+            a = a + pa1 * S[j][1]; b = b + pb1 * S[j][1];
+            a = a + pa2 * S[j][2]; b = b + pb2 * S[j][2];
+            a = a + pa3 * S[j][3]; b = b + pb3 * S[j][3];
+            f1 = f1 + a * U1.v[j] + b * Uout[j];
+            f2 = f2 + a * U2.v[j] + b * Uout[j];
         }
-        const double g1 = 0.5 * lent * f1, g2 = 0.5 * lent * f2;
-        if (i == 0) { r1.v[0] = g1; r2.v[0] = g2; }
-        else if (i == 1) { r1.v[1] = g1; r2.v[1] = g2; }
-        else if (i == 2) { r1.v[2] = g1; r2.v[2] = g2; }
-        else { r1.v[3] = g1; r2.v[3] = g2; }
+        r1.v[i] = 0.5 * lent * f1;
+        r2.v[i] = 0.5 * lent * f2;
     }
 }
 

@@ -56,8 +56,10 @@ __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, i
     const int work = (NT * (visc ? 2 : 1) * 8) > ((NE + 2 * NB) * 32) ? NT * (visc ? 2 : 1) * 8
                                                                       : (NE + 2 * NB) * 32;
     const int index = 4 * (NG + 1) + 4 * (NP + 1) + 2 * (NGTL + NPSL);
+    // ais of G, gathered with the state
+    const int pre = ((NG + 1) & ~1) * 8;
     // gradients: FP64, or FP32 in the split mode (its reconstruction reads FP32)
-    return NLa * 48 + (visc ? NLa * 24 : 0) + NG * (split ? 32 : 64) + work + ((index + 15) & ~15);
+    return NLa * 48 + (visc ? NLa * 24 : 0) + NG * (split ? 32 : 64) + pre + work + ((index + 15) & ~15);
 }
 
 }  // namespace fvm

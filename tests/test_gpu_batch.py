@@ -119,3 +119,30 @@ def test_batch_viscous_same_as_alone(name, precision):
             np.testing.assert_array_equal(got["nut"], r["nut"])
     for s in solvers:
         s.close()
+
+
+def test_batch_member_error_is_isolated():
+    """A member that hits a reference STOP condition (here a negative
+    pressure put into its state) stops with the status and index it gets
+    alone; the other members run on, bit for bit as alone."""
+    (ga, ka), (gb, kb), _ = euler_members()
+    ka, kb = dict(EULER, **ka), dict(EULER, **kb)
+    with Solver(ga, Inputs.from_kwargs(**ka)) as a:
+        prim = a.get_state()["prim"].copy()
+    prim[4 * 400 + 3] = -50.0                   # pressure of vertex 401
+    alone_err = None
+    with Solver(ga, Inputs.from_kwargs(**ka)) as a:
+        a.set_state(prim)
+        with pytest.raises(FvmError) as ei:
+            a.iterate(20)
+        alone_err = (ei.value.code, ei.value.index, a.monitor()[0]["iter"])
+    b_ref = alone(gb, kb, 20)[1]["prim"]
+    with Solver(ga, Inputs.from_kwargs(**ka)) as a, Solver(gb, Inputs.from_kwargs(**kb)) as b, \
+            Batch() as batch:
+        a.set_state(prim)
+        batch.add(a)
+        batch.add(b)
+        st = batch.iterate(20)
+        assert st[0] == alone_err[0] < 0 and st[1] == 0
+        assert a.monitor()[0]["iter"] == alone_err[2]
+        np.testing.assert_array_equal(b.get_state()["prim"], b_ref)
