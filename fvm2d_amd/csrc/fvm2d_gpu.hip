@@ -45,6 +45,9 @@ constexpr int kBlock = 256;
 // waves per SIMD the stage kernel is compiled for (one workgroup of
 // kBlock threads puts one wave on each SIMD of a CU)
 constexpr int kPatchWavesEuler = 4, kPatchWavesVisc = 3, kPatchWavesSplit = 4;
+// patch vertices whose viscous stress terms groups of three lanes can take
+// (21 groups a wavefront)
+constexpr int kViscGroups = 21 * (kBlock / 64);
 constexpr int kHistCap = 4096;      // history rows per fvm_gpu_iterate chunk
 constexpr int kGraphIters = 8;      // iterations captured per graph
 
@@ -470,7 +473,7 @@ __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int
 // condition (src/main.f:42, :67-75).  One wide workgroup, so the partials and
 // the solid edges are each read in one pass; fixed reduction order.  Without
 // viscosity the stress terms are zero and are left out.
-constexpr int kFinal = 1024;
+constexpr int kFinal = 512;
 
 // force of solid edge b on (xf, yf) from the wall terms its two end points
 // left (src/clcd.f:18-43)
@@ -663,13 +666,17 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     // LDS carve-up for this patch
     double4 *s_prim = reinterpret_cast<double4 *>(smem);
     double2 *s_coord = reinterpret_cast<double2 *>(s_prim + NLa);
+    // viscosity per local vertex.  SA: laminar (s_mul), total (s_mu) and
+    // nu_tilde (s_nut).  Laminar: s_mul is s_mu, and s_e holds p/rho, which
+    // the stress terms would otherwise divide out per triangle.
     double *s_mu = reinterpret_cast<double *>(s_coord + NLa);
-    double *s_mul = s_mu + (VISC ? NLa : 0);
-    double *s_nut = s_mul + (VISC ? NLa : 0);
+    double *s_mul = TURB ? s_mu + NLa : s_mu;
+    double *s_nut = s_mul + (TURB ? NLa : 0);
+    double *s_e = s_mu + (VISC ? NLa : 0);
     // gradients in LDS: FP64, or rounded to FP32 in the split mode, whose
     // reconstruction reads them in FP32 (the FP64 values go to qx, qy)
     using GT = typename std::conditional<PREC == 1, float4, double4>::type;
-    GT *s_gx = reinterpret_cast<GT *>(s_nut + (VISC ? NLa : 0));
+    GT *s_gx = reinterpret_cast<GT *>(s_mu + NLa * (TURB ? 3 : (VISC ? 2 : 0)));
     GT *s_gy = s_gx + NG;
     // the gradients' area factor ais of G, gathered with the state so that
     // the gradient phase reads no global memory
@@ -697,9 +704,12 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         if (i < NG) s_ais[i] = d.ais[gv];
         if (VISC) {
             double ml = sutherland_viscosity(c, p.x, p.w);
-            double nu = d.nut[gv];
-            double m = TURB ? ml + mu_turb(c, ml, p.x, nu) : ml;
-            s_mul[i] = ml; s_mu[i] = m; s_nut[i] = nu;
+            if (TURB) {
+                double nu = d.nut[gv];
+                s_mul[i] = ml; s_mu[i] = ml + mu_turb(c, ml, p.x, nu); s_nut[i] = nu;
+            } else {
+                s_mu[i] = ml; s_e[i] = p.w / p.x;
+            }
         }
     }
     for (int i = t; i <= NG; i += kBlock) s_gptr[i] = gptr[i];
@@ -741,6 +751,53 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     // terms in ascending triangle order; with viscosity a vertex of P also
     // sums the triangles' Galerkin stress terms into its residual
     // (src/green_gauss.f:132-194), evaluated here from LDS.
+    //
+    // The stress term of triangle e (position m of the vertex) for one vertex
+    auto visc_term = [&](const int e, const short4 tr, const V4 &p1, const V4 &p2, const V4 &p3,
+                         const double *bx, const double *by, const double *dxt, const double *dyt,
+                         double &c1, double &c2, double &c3) {
+        const double n2b3 = 2.0 / 3.0;
+        double e1, e2, e3;
+        if (TURB) {
+            e1 = p1.v[3] / p1.v[0]; e2 = p2.v[3] / p2.v[0]; e3 = p3.v[3] / p3.v[0];
+        } else {
+            e1 = s_e[tr.x]; e2 = s_e[tr.y]; e3 = s_e[tr.z];
+        }
+        double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
+        double eyt = e1 * by[0] + e2 * by[1] + e3 * by[2];
+        double um = (p1.v[1] + p2.v[1] + p3.v[1]) / 3.0;
+        double vm = (p1.v[2] + p2.v[2] + p3.v[2]) / 3.0;
+        double mut = (s_mu[tr.x] + s_mu[tr.y] + s_mu[tr.z]) / 3.0;
+        double txx = n2b3 * mut * (2.0 * dxt[1] - dyt[2]);
+        double txy = mut * (dyt[1] + dxt[2]);
+        double tyy = n2b3 * mut * (2.0 * dyt[2] - dxt[1]);
+        double gpr;
+        if (TURB) {
+            double mult = (s_mul[tr.x] + s_mul[tr.y] + s_mul[tr.z]) / 3.0;
+            double prim5t = (s_nut[tr.x] + s_nut[tr.y] + s_nut[tr.z]) / 3.0;
+            double rhot = (p1.v[0] + p2.v[0] + p3.v[0]) / 3.0;
+            double chi = prim5t * rhot / mult;
+            double chi3 = pow3(chi);
+            double fv1 = chi3 / (chi3 + c.Cv11);
+            gpr = c.GAMMA * (mult / c.prandtl + rhot * prim5t * fv1 / c.prandtl_turb) / c.GAMMA1;
+        } else {
+            // nu_tilde is zero, so the eddy term is +0 and the laminar
+            // viscosity is the total one: the same value, fewer operations
+            gpr = c.GAMMA * (mut / c.prandtl) / c.GAMMA1;
+        }
+        double tx = txx * um + txy * vm + gpr * ext;
+        double ty = txy * um + tyy * vm + gpr * eyt;
+        double areat = 0.25 / s_larea[e >> 2];
+        const int m = e & 3;
+        const double bxm = bx[m], bym = by[m];
+        c1 = (txx * bxm + txy * bym) * areat;
+        c2 = (txy * bxm + tyy * bym) * areat;
+        c3 = (tx * bxm + ty * bym) * areat;
+    };
+    // With viscosity and a patch of at most kViscGroups vertices, a vertex's
+    // stress terms are evaluated by a group of three lanes (below); larger
+    // patches add them in the gradient loop.
+    const bool grouped = VISC && NP <= kViscGroups;
     double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
     for (int g = t; g < NG; g += kBlock) {
         const double ais = s_ais[g];
@@ -749,32 +806,12 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
                            const double *bx, const double *by, const double *dxt, const double *dyt) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) { gx[i] = gx[i] + dxt[i]; gy[i] = gy[i] + dyt[i]; }
-            if (VISC && g < NP) {
-                const double n2b3 = 2.0 / 3.0;
-                double e1 = p1.v[3] / p1.v[0], e2 = p2.v[3] / p2.v[0], e3 = p3.v[3] / p3.v[0];
-                double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
-                double eyt = e1 * by[0] + e2 * by[1] + e3 * by[2];
-                double um = (p1.v[1] + p2.v[1] + p3.v[1]) / 3.0;
-                double vm = (p1.v[2] + p2.v[2] + p3.v[2]) / 3.0;
-                double mut = (s_mu[tr.x] + s_mu[tr.y] + s_mu[tr.z]) / 3.0;
-                double mult = (s_mul[tr.x] + s_mul[tr.y] + s_mul[tr.z]) / 3.0;
-                double prim5t = (s_nut[tr.x] + s_nut[tr.y] + s_nut[tr.z]) / 3.0;
-                double rhot = (p1.v[0] + p2.v[0] + p3.v[0]) / 3.0;
-                double chi = prim5t * rhot / mult;
-                double chi3 = pow3(chi);
-                double fv1 = chi3 / (chi3 + c.Cv11);
-                double txx = n2b3 * mut * (2.0 * dxt[1] - dyt[2]);
-                double txy = mut * (dyt[1] + dxt[2]);
-                double tyy = n2b3 * mut * (2.0 * dyt[2] - dxt[1]);
-                double gpr = c.GAMMA * (mult / c.prandtl + rhot * prim5t * fv1 / c.prandtl_turb) / c.GAMMA1;
-                double tx = txx * um + txy * vm + gpr * ext;
-                double ty = txy * um + tyy * vm + gpr * eyt;
-                double areat = 0.25 / s_larea[e >> 2];
-                const int m = e & 3;
-                const double bxm = bx[m], bym = by[m];
-                rv1 = rv1 + (txx * bxm + txy * bym) * areat;
-                rv2 = rv2 + (txy * bxm + tyy * bym) * areat;
-                rv3 = rv3 + (tx * bxm + ty * bym) * areat;
+            if (VISC && !grouped && g < NP) {
+                double c1, c2, c3;
+                visc_term(e, tr, p1, p2, p3, bx, by, dxt, dyt, c1, c2, c3);
+                rv1 = rv1 + c1;
+                rv2 = rv2 + c2;
+                rv3 = rv3 + c3;
             }
         };
         const int k1 = s_gptr[g + 1];
@@ -796,6 +833,39 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             s_gy[g] = GT{qy.x, qy.y, qy.z, qy.w};
         }
         if (MODE != 0 && g < NP) { d.qx[v0 + g] = qx; d.qy[v0 + g] = qy; }
+    }
+    // Stress terms in groups of three lanes: lane j of vertex v's group
+    // evaluates its triangles j, j + 3, ...; the running sums go round the
+    // group (lane 0, 1, 2, 0, ...), each lane adding its term in triangle
+    // order, so they are the sums of the loop above, bit for bit.  21 groups
+    // a wavefront, filled from the last wavefront, which the gradients leave
+    // idle; the sums end in the group's first lane, which then updates v.
+    int tl = t;                                   // local vertex this thread updates
+    if (grouped) {
+        const int w = t >> 6, lane = t & 63, gs = lane / 3, j = lane - 3 * gs;
+        const int v = gs < 21 ? 21 * (kBlock / 64 - 1 - w) + gs : NP;
+        const bool act = v < NP;
+        const int kb0 = act ? s_gptr[v] : 0, cnt = act ? s_gptr[v + 1] - kb0 : 0;
+        const int prev = 3 * gs + (j == 0 ? 2 : j - 1);   // lane before this one in the group
+        for (int q = 0; __any(3 * q < cnt); ++q) {
+            const int k = 3 * q + j;
+            const bool have = k < cnt;
+            double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+            if (have) {
+                const int e = s_gtl[kb0 + k];
+                const short4 tr = s_ltri[e >> 2];
+                V4 p1, p2, p3;
+                double bx[3], by[3], dxt[4], dyt[4];
+                tri_gg(s_coord, s_prim, tr, p1, p2, p3, bx, by, dxt, dyt);
+                visc_term(e, tr, p1, p2, p3, bx, by, dxt, dyt, c1, c2, c3);
+            }
+#pragma unroll
+            for (int step = 0; step < 3; ++step) {
+                if (j == step && have) { rv1 = rv1 + c1; rv2 = rv2 + c2; rv3 = rv3 + c3; }
+                rv1 = __shfl(rv1, prev); rv2 = __shfl(rv2, prev); rv3 = __shfl(rv3, prev);
+            }
+        }
+        tl = (act && j == 0 && gs < 21) ? v : NP;
     }
     __syncthreads();
 
@@ -850,8 +920,8 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
 
     // 4. residual of P in the reference's order, then the RK stage
     Red red{0.0, 0.0, 0};
-    if (t < NP) {
-        const int v = v0 + t;
+    if (tl < NP) {
+        const int v = v0 + tl;
         // loads of the update, issued before the ordered sum
         double4 o = make_double4(0.0, 0.0, 0.0, 0.0);
         double dtv = 0.0, cva = 1.0;
@@ -859,7 +929,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         V4 r;
         r.v[0] = 0.0;
         r.v[1] = VISC ? rv1 : 0.0; r.v[2] = VISC ? rv2 : 0.0; r.v[3] = VISC ? rv3 : 0.0;
-        const int k1 = s_pptr[t + 1];
+        const int k1 = s_pptr[tl + 1];
         const double4 *fl = reinterpret_cast<const double4 *>(s_work);
         auto apply = [&](const int q, const double4 f) {
             if (q & 1) {
@@ -869,7 +939,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             }
         };
         // two terms a step, read before either is added, added in order
-        int k = s_pptr[t];
+        int k = s_pptr[tl];
         for (; k + 1 < k1; k += 2) {
             const int qa = s_psl[k], qb = s_psl[k + 1];
             const double4 fa = fl[qa >> 1], fb = fl[qb >> 1];
@@ -882,11 +952,11 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         }
         // mu, mul, qx, qy of the last residual of an iteration are what the
         // monitors, the SA step and the caller read
-        if (VISC && MODE != 0) { d.mul[v] = s_mul[t]; d.mu[v] = s_mu[t]; }
+        if (VISC && MODE != 0) { d.mul[v] = s_mul[tl]; d.mu[v] = s_mu[tl]; }
         if (MODE == 2) {
             st4(res_out, v, r);
         } else {
-            const double4 pc = s_prim[t];            // o: state at the start of the iteration
+            const double4 pc = s_prim[tl];           // the stage state
             V4 po{{o.x, o.y, o.z, o.w}}, p{{pc.x, pc.y, pc.z, pc.w}}, u0, u1, un;
             prim2con(c, po, u0);
             prim2con(c, p, u1);
@@ -903,7 +973,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             pout[v] = q;
             if (MODE == 1) {
                 if (d.solid[v])
-                    wall_terms<VISC>(d, v, q.w, VISC ? s_mu[t] : 0.0, d.qx[v], d.qy[v]);
+                    wall_terms<VISC>(d, v, q.w, VISC ? s_mu[tl] : 0.0, d.qx[v], d.qy[v]);
                 const double gra = q.x - o.x;
                 const double dif = fabs(gra);
                 red.s = sq(gra);
@@ -918,7 +988,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
                     double aa = sqrt(c.GAMMA * q.w / q.x);
                     double dr = d.drmin[v];
                     if (VISC) {
-                        double dtv = 2.0 * c.GAMMA * s_mu[t] / (q.x * c.prandtl);
+                        double dtv = 2.0 * c.GAMMA * s_mu[tl] / (q.x * c.prandtl);
                         d.dt[v] = c.cfl * sq(dr) / (dr * (qq + aa) + dtv);
                     } else {
                         d.dt[v] = c.cfl * dr / (qq + aa);
@@ -1678,7 +1748,7 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
                   const std::vector<int> &v2t, const std::vector<int4> &tri,
                   const std::vector<int> &v2e_ptr, const std::vector<int> &v2e,
                   const std::vector<int2> &enode, const std::vector<int> &v2b_ptr,
-                  const std::vector<int> &v2b, const std::vector<int2> &bnode, bool visc,
+                  const std::vector<int> &v2b, const std::vector<int2> &bnode, int iflow,
                   bool split, int lds_limit, PatchHost &ph)
 {
     const int ntri = (int)tri.size(), nedge = (int)enode.size(), nbnd = (int)bnode.size();
@@ -1774,7 +1844,7 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
         if (NL > 32767 || 4 * (int)tl.size() > 32767 || 2 * (NE + 2 * NB) > 32767) return FVM_E_SIZE;
         h[PH_NGTL] = (int)ph.gtl.size() - h[PH_GTL];
         h[PH_NPSL] = (int)ph.pslot.size() - h[PH_PSL];
-        const int lds = patch_lds_bytes(NP, NL, NG, (int)tl.size(), NE, NB, h[PH_NGTL], h[PH_NPSL], visc,
+        const int lds = patch_lds_bytes(NP, NL, NG, (int)tl.size(), NE, NB, h[PH_NGTL], h[PH_NPSL], iflow,
                                         split);
         ph.lds_max = std::max(ph.lds_max, lds);
         for (int i = 0; i < kPatchHdr; ++i) ph.hdr.push_back(h[i]);
@@ -1899,7 +1969,7 @@ int build_layout(const fvm_mesh *m, int iflow, int precision, int patch_size, in
     for (;;) {
         L.ph = PatchHost();
         int rc = build_patches(np, part.patch_ptr, nadj_ptr, nadj, v2t_ptr, v2t, tri, v2e_ptr, v2e,
-                               enode, v2b_ptr, v2b, bnode, iflow != 1, precision == 1, lds_limit,
+                               enode, v2b_ptr, v2b, bnode, iflow, precision == 1, lds_limit,
                                L.ph);
         if (rc == FVM_OK) break;
         if (rc != FVM_E_SIZE || target <= 16) return rc;
