@@ -104,8 +104,9 @@ def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
     reqs = serving.workload(48, 2024, np_min=1e3, np_max=2e4)
     cfg = serving.ServeConfig(max_iters=10**6, minres=0.0)
     prepared = [(q, *serving.prepare(q, cfg)) for q in reqs]
-    # plain FP64 path, one request at a time (iteration time only)
-    c64 = serving.ServeConfig(max_iters=10**6, minres=0.0, precision=0)
+    # plain FP64 path, one request at a time, with the library's automatic
+    # patch size (iteration time only)
+    c64 = serving.ServeConfig(max_iters=10**6, minres=0.0, precision=0, patch_size=0)
     vi = 0.0
     tp = 0.0
     from fvm2d_amd.gpu import FvmError

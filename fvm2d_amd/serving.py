@@ -124,13 +124,18 @@ class ServeConfig:
     prep_slice: float = 0.02          # seconds of host preparation per engine step
     cfl: float = 1.0
     first_spacing: float = 1e-3       # wall spacing of the O-grids (config 0 construction)
+    # vertices per patch of a served solve: a batch fills the GPU with many
+    # solves, so the patches are sized for little halo work rather than for
+    # filling the GPU with one small mesh (0: the library's automatic size)
+    patch_size: int = 64
 
 
 def inputs_for(req: Request, cfg: ServeConfig) -> Inputs:
     return Inputs(iflow=3 if req.flow == "rans" else 1, iflux=1, ilimit=1,
                   mach_inf=req.mach, aoa_deg=req.aoa, rey=req.rey, cfl=cfg.cfl,
                   iterlast=0, maxiter=cfg.max_iters, minres=cfg.minres,
-                  saveinterval=0, precision=cfg.precision, nut_factor=3.0, sa_compat=0)
+                  saveinterval=0, precision=cfg.precision, nut_factor=3.0, sa_compat=0,
+                  patch_size=cfg.patch_size)
 
 
 def prepare(req: Request, cfg: ServeConfig):
