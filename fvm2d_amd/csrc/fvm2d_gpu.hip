@@ -1121,15 +1121,6 @@ __global__ void __launch_bounds__(kBlock) k_tstep_batch(const BatchMember *__res
     tstep_body<VISC>(M.d, M.c, w.y * kBlock + threadIdx.x);
 }
 
-template <bool COMPAT>
-__global__ void __launch_bounds__(kBlock) k_sa_batch(const BatchMember *__restrict__ mem,
-                                                     const int2 *__restrict__ vwork)
-{
-    const int2 w = uniform_at(vwork, (int)blockIdx.x);
-    const BatchMember &M = uniform_at(mem, w.x);
-    sa_body<COMPAT>(M.d, M.c, M.S, w.y * kBlock + threadIdx.x);
-}
-
 __global__ void __launch_bounds__(kBlock) k_copy_batch(const BatchMember *__restrict__ mem,
                                                        const int2 *__restrict__ vwork)
 {
@@ -1141,6 +1132,150 @@ __global__ void __launch_bounds__(kBlock) k_copy_batch(const BatchMember *__rest
 }
 
 // one wide workgroup per member
+// SA step over patches (src/sa_model.f): sa_body with its gathers served
+// from LDS.  A workgroup loads, for its patch P and 1-ring (G), the state
+// after the RK stages, nu_tilde, the laminar viscosity and the coordinates,
+// and the local triangle, edge and boundary-edge tables; each vertex of P
+// then makes sa_body's sums in sa_body's order (triangles ascending, edges
+// in edge order, farfield edges in boundary order), so nu_tilde comes out
+// bit for bit the same.
+template <bool COMPAT>
+__device__ __forceinline__ void sa_patch_body(const Dev &d, const Consts &c, DevState *S,
+                                              const PatchArgs &pa, const int b)
+{
+    const int stop = S->done | S->err_code;
+    const int hb = kPatchHdr * b;
+    auto hdr = [&](int k) { return uniform_at(pa.hdr, hb + k); };
+    const int v0 = hdr(PH_V0), NP = hdr(PH_NP), NG = hdr(PH_NG);
+    const int NT = hdr(PH_NTRI), NE = hdr(PH_NEDGE), NB = hdr(PH_NBND);
+    const int NGTL = hdr(PH_NGTL), NPSL = hdr(PH_NPSL);
+    const int *halo = pa.halo + hdr(PH_HALO);
+    const short4 *ltri = pa.ltri + hdr(PH_TRI);
+    const double *larea = pa.larea + hdr(PH_TRI);
+    const int *gptr = pa.gptr + hdr(PH_GPTR);
+    const short *gtl = pa.gtl + hdr(PH_GTL);
+    const short2 *ledge = pa.ledge + hdr(PH_EDGE);
+    const double2 *ledge_ds = pa.ledge_ds + hdr(PH_EDGE);
+    const short2 *lbnd = pa.lbnd + hdr(PH_BND);
+    const int *lbnd_id = pa.lbnd_id + hdr(PH_BND);
+    const int *pptr = pa.pptr + hdr(PH_PPTR);
+    const short *pslot = pa.pslot + hdr(PH_PSL);
+    const int t = threadIdx.x;
+    extern __shared__ double4 smem4[];
+    const int NGa = (NG + 1) & ~1, NTa = (NT + 1) & ~1;
+    double4 *s_prim = smem4;
+    double2 *s_coord = reinterpret_cast<double2 *>(s_prim + NG);
+    double *s_nut = reinterpret_cast<double *>(s_coord + NG);
+    double *s_mul = s_nut + NGa;
+    double *s_larea = s_mul + NGa;
+    double2 *s_eds = reinterpret_cast<double2 *>(s_larea + NTa);
+    short4 *s_ltri = reinterpret_cast<short4 *>(s_eds + NE);
+    short2 *s_ledge = reinterpret_cast<short2 *>(s_ltri + NT);
+    short2 *s_lbnd = s_ledge + NE;
+    int *s_lbid = reinterpret_cast<int *>(s_lbnd + NB);
+    int *s_gptr = s_lbid + NB;
+    int *s_pptr = s_gptr + NP + 1;
+    short *s_gtl = reinterpret_cast<short *>(s_pptr + NP + 1);
+    short *s_psl = s_gtl + NGTL;
+
+    for (int i = t; i < NG; i += kBlock) {
+        const int gv = i < NP ? v0 + i : halo[i - NP];
+        s_prim[i] = d.prim[gv];
+        s_coord[i] = d.coord[gv];
+        s_nut[i] = d.nut[gv];
+        s_mul[i] = d.mul[gv];
+    }
+    for (int i = t; i < NT; i += kBlock) { s_ltri[i] = ltri[i]; s_larea[i] = larea[i]; }
+    for (int i = t; i < NE; i += kBlock) { s_ledge[i] = ledge[i]; s_eds[i] = ledge_ds[i]; }
+    for (int i = t; i < NB; i += kBlock) { s_lbnd[i] = lbnd[i]; s_lbid[i] = lbnd_id[i]; }
+    for (int i = t; i <= NP; i += kBlock) { s_gptr[i] = gptr[i]; s_pptr[i] = pptr[i]; }
+    for (int i = t; i < NGTL; i += kBlock) s_gtl[i] = gtl[i];
+    for (int i = t; i < NPSL; i += kBlock) s_psl[i] = pslot[i];
+    __syncthreads();
+    if (stop || t >= NP) return;
+
+    const int v = v0 + t;
+    // loads of the update, issued before the sums
+    const double wdv = d.wd[v], dti = d.dt[v], cva = d.cvarea[v];
+    const double Omega = d.qy[v].y - d.qx[v].z;
+    const bool solid = d.solid[v] != 0;
+    const double4 p = s_prim[t];
+    const double nu = s_nut[t], mulv = s_mul[t];
+    double g1c = 0.0, g2c = 0.0, g3c = 0.0;
+    if (COMPAT) { g1c = d.nut[d.sa1]; g2c = d.nut[d.sa2]; g3c = d.nut[d.sa3]; }
+    double divf = 0.0;
+    for (int k = s_gptr[t]; k < s_gptr[t + 1]; ++k) {
+        const int e = s_gtl[k];
+        const short4 n = s_ltri[e >> 2];
+        const double2 x1 = s_coord[n.x], x2 = s_coord[n.y], x3 = s_coord[n.z];
+        const double areat = 0.25 / s_larea[e >> 2];
+        const double bx[3] = {x2.y - x3.y, x3.y - x1.y, x1.y - x2.y};
+        const double by[3] = {x3.x - x2.x, x1.x - x3.x, x2.x - x1.x};
+        const double n1 = s_nut[n.x], n2 = s_nut[n.y], n3 = s_nut[n.z];
+        const double g1 = COMPAT ? g1c : n1, g2 = COMPAT ? g2c : n2, g3 = COMPAT ? g3c : n3;
+        const double nux = g1 * bx[0] + g2 * bx[1] + g3 * bx[2];
+        const double nuy = g1 * by[0] + g2 * by[1] + g3 * by[2];
+        const double mult = (s_mul[n.x] + s_mul[n.y] + s_mul[n.z]) / 3.0;
+        const double prim5t = (n1 + n2 + n3) / 3.0;
+        const double rhot = (s_prim[n.x].x + s_prim[n.y].x + s_prim[n.z].x) / 3.0;
+        const double mutb = mult / rhot + prim5t;
+        const int m = e & 3;
+        divf = divf + (c.Cb2Sig1 * mutb - c.Cb2Sig2 * (mulv / p.x + nu)) * (nux * bx[m] + nuy * by[m]) * areat;
+    }
+    for (int k = s_pptr[t]; k < s_pptr[t + 1]; ++k) {
+        const int q = s_psl[k], term = q >> 1;
+        if (term < NE) {
+            const short2 nn = s_ledge[term];
+            const double2 ds = s_eds[term];
+            const double4 pa_ = s_prim[nn.x], pb = s_prim[nn.y];
+            const double unl = pa_.y * ds.x + pa_.z * ds.y;
+            const double unr = pb.y * ds.x + pb.z * ds.y;
+            const double una = 0.5 * (unl + unr);
+            const double flux = (una > 0.0) ? una * s_nut[nn.x] : una * s_nut[nn.y];
+            divf = (q & 1) ? divf - flux : divf + flux;
+        } else {
+            const int kb = (term - NE) >> 1, side = (term - NE) & 1;
+            if (s_lbid[kb] < d.nsw) continue;
+            const short2 nn = s_lbnd[kb];
+            const double2 x1 = s_coord[nn.x], x2 = s_coord[nn.y];
+            const double sx = x2.y - x1.y, sy = -(x2.x - x1.x);
+            const double u = 0.5 * (s_prim[nn.x].y + s_prim[nn.y].y), w = 0.5 * (s_prim[nn.x].z + s_prim[nn.y].z);
+            const double un = u * sx + w * sy;
+            const double fl = (un > 0.0) ? un * s_nut[side ? nn.y : nn.x] : un * c.nut_inf;
+            divf = divf + 0.5 * fl;
+        }
+    }
+    const double chi = nu * p.x / mulv;
+    const double chi3 = pow3(chi);
+    const double fv1 = chi3 / (chi3 + c.Cv11);
+    const double fv2 = 1.0 / pow3(1.0 + chi / c.Cv2);
+    double r = nu / wdv / (wdv * c.kolm2);
+    const double fv3 = (1.0 + chi * fv1) * (1.0 - fv2) / fmax(chi, 0.001);
+    const double Sv = fv3 * fabs(Omega) + r * fv2;
+    r = r / Sv;
+    r = fmin(r, 2.0);
+    const double g = r + c.Cw2 * (pow6(r) - r);
+    const double fw = g * pow(c.Cw31 / (pow6(g) + c.Cw32), 1.0 / 6.0);
+    const double source = c.Cb1 * Sv * nu;
+    const double fact = 1.0 + c.Cw1 * fw * (nu / wdv) * (dti / wdv);
+    double nn = (nu - (dti / cva) * divf + dti * source) / fact;
+    if (solid) nn = 0.0;
+    d.nut_new[v] = nn;
+    if (p.x <= 0.0 || p.w <= 0.0 || nn < 0.0) {
+        atomicMin(&S->err_index, d.orig[v]);
+        S->err_code = FVM_E_NONPOSITIVE;
+    }
+}
+
+template <bool COMPAT>
+__global__ void __launch_bounds__(kBlock) k_sa_patch_batch(const BatchMember *__restrict__ mem,
+                                                           const int2 *__restrict__ work, int nwork, int xcd)
+{
+    const int2 w = uniform_at(work, logical_block(nwork, xcd));
+    const BatchMember &M = uniform_at(mem, w.x);
+    sa_patch_body<COMPAT>(M.d, M.c, M.S, M.pa, w.y);
+}
+
 template <bool VISC>
 __global__ void __launch_bounds__(kFinal) k_final_batch(const BatchMember *__restrict__ mem)
 {
@@ -1208,7 +1343,7 @@ struct fvm_batch {
     BatchMember *d_mem = nullptr;
     int2 *d_work = nullptr, *d_vwork = nullptr;
     DevState *d_states = nullptr;
-    int nwork = 0, nvwork = 0, lds = 0, max_nff = 0;
+    int nwork = 0, nvwork = 0, lds = 0, sa_lds = 0, max_nff = 0;
     bool dirty = true, any_vortex = false;
     hipGraphExec_t graph_n = nullptr, graph_1 = nullptr;
     bool graphs_ok = true;
@@ -1576,8 +1711,19 @@ void one_iteration(fvm_ctx *x, Tally *tally)
     }
     if (turb) {
         timed("k_sa", [&] {
-            if (x->par.sa_compat) hipLaunchKernelGGL(k_sa<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c, x->S);
-            else hipLaunchKernelGGL(k_sa<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c, x->S);
+            if (x->variant == 3) {
+                const int n = x->pa.npatch, lds = x->pa.sa_lds_bytes;
+                if (x->par.sa_compat)
+                    hipLaunchKernelGGL(k_sa_patch_batch<true>, dim3(n), dim3(kBlock), lds, x->stream,
+                                       x->self_mem, x->self_work, n, x->d.xcd);
+                else
+                    hipLaunchKernelGGL(k_sa_patch_batch<false>, dim3(n), dim3(kBlock), lds, x->stream,
+                                       x->self_mem, x->self_work, n, x->d.xcd);
+            } else if (x->par.sa_compat) {
+                hipLaunchKernelGGL(k_sa<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c, x->S);
+            } else {
+                hipLaunchKernelGGL(k_sa<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c, x->S);
+            }
             hipLaunchKernelGGL(k_copy, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d.nut_new, x->d.nut, np, x->S);
         });
     }
@@ -1737,6 +1883,7 @@ struct PatchHost {
     std::vector<double2> ledge_ds;
     std::vector<double> larea;
     int lds_max = 0;
+    int sa_lds_max = 0;
 };
 
 // dynamic LDS per workgroup when the device is not asked: 64 KiB minus the
@@ -1847,6 +1994,8 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
         const int lds = patch_lds_bytes(NP, NL, NG, (int)tl.size(), NE, NB, h[PH_NGTL], h[PH_NPSL], iflow,
                                         split);
         ph.lds_max = std::max(ph.lds_max, lds);
+        ph.sa_lds_max = std::max(ph.sa_lds_max, sa_lds_bytes(NP, NG, (int)tl.size(), NE, NB, h[PH_NGTL],
+                                                               h[PH_NPSL]));
         for (int i = 0; i < kPatchHdr; ++i) ph.hdr.push_back(h[i]);
         for (int v : lv) loc[v] = -1;
         for (int t : tl) tloc[t] = -1;
@@ -2262,6 +2411,7 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     TRY(upload(x, (short **)&x->pa.pslot, L.ph.pslot));
     x->pa.npatch = (int)L.patch_ptr.size() - 1;
     x->pa.lds_bytes = L.ph.lds_max;
+    x->pa.sa_lds_bytes = L.ph.sa_lds_max;
     x->target = L.target;
     {
         long long halo = 0; int maxp = 0;
@@ -2612,7 +2762,7 @@ int batch_rebuild(fvm_batch *b)
     batch_free(b);
     std::vector<BatchMember> m;
     std::vector<int2> work, vwork;
-    b->lds = 0; b->max_nff = 0; b->any_vortex = false;
+    b->lds = 0; b->sa_lds = 0; b->max_nff = 0; b->any_vortex = false;
     std::vector<int> cost;
     for (int i = 0; i < (int)b->mem.size(); ++i) {
         const fvm_ctx *x = b->mem[i];
@@ -2623,6 +2773,7 @@ int batch_rebuild(fvm_batch *b)
         }
         for (int k = 0; k < (int)nblk(x->np); ++k) vwork.push_back(make_int2(i, k));
         b->lds = std::max(b->lds, x->pa.lds_bytes);
+        b->sa_lds = std::max(b->sa_lds, x->pa.sa_lds_bytes);
         b->max_nff = std::max(b->max_nff, x->nff);
         if (x->c.vortex == 1 && x->c.mach_inf < 1.0) b->any_vortex = true;
     }
@@ -2653,9 +2804,11 @@ void batch_iteration(fvm_batch *b)
     for (int irk = 1; irk <= 3; ++irk) launch_stage(L, irk, irk == 3 ? 1 : 0);
     if (turb) {
         if (q.sa_compat)
-            hipLaunchKernelGGL(k_sa_batch<true>, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
+            hipLaunchKernelGGL(k_sa_patch_batch<true>, dim3(b->nwork), dim3(kBlock), b->sa_lds, b->stream,
+                               b->d_mem, b->d_work, b->nwork, 1);
         else
-            hipLaunchKernelGGL(k_sa_batch<false>, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
+            hipLaunchKernelGGL(k_sa_patch_batch<false>, dim3(b->nwork), dim3(kBlock), b->sa_lds, b->stream,
+                               b->d_mem, b->d_work, b->nwork, 1);
         hipLaunchKernelGGL(k_copy_batch, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
     }
     if (q.iflow == 1)

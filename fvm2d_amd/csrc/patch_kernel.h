@@ -46,6 +46,7 @@ struct PatchArgs {
     const short *pslot;     // 2 * (term index) + (1 if subtracted)
     int npatch;
     int lds_bytes;
+    int sa_lds_bytes;       // of the SA-step kernel over the same patches
 };
 
 // bytes of the LDS carve-up of one patch; the kernel and the planner both use it
@@ -64,6 +65,16 @@ __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, i
     // gradients: FP64, or FP32 in the split mode (its reconstruction reads FP32)
     return NLa * 48 + NLa * 8 * (flow == 3 ? 3 : (visc ? 2 : 0)) + NG * (split ? 32 : 64) + pre + work +
            ((index + 15) & ~15);
+}
+
+// bytes of the LDS carve-up of the SA step over one patch: state,
+// coordinates, nu_tilde and laminar viscosity of G; the local triangles with
+// their areas; the edge and boundary-edge tables; the per-vertex lists
+__host__ __device__ inline int sa_lds_bytes(int NP, int NG, int NT, int NE, int NB, int NGTL, int NPSL)
+{
+    const int NGa = (NG + 1) & ~1, NTa = (NT + 1) & ~1;
+    return NG * 48 + NGa * 16 + NTa * 8 + NE * 16 + NT * 8 + NE * 4 + NB * 8 + 8 * (NP + 1) +
+           2 * (NGTL + NPSL);
 }
 
 }  // namespace fvm
