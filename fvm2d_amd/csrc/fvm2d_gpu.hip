@@ -666,17 +666,17 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     // LDS carve-up for this patch
     double4 *s_prim = reinterpret_cast<double4 *>(smem);
     double2 *s_coord = reinterpret_cast<double2 *>(s_prim + NLa);
-    // viscosity per local vertex.  SA: laminar (s_mul), total (s_mu) and
-    // nu_tilde (s_nut).  Laminar: s_mul is s_mu, and s_e holds p/rho, which
-    // the stress terms would otherwise divide out per triangle.
+    // per local vertex: viscosity and p/rho (which the stress terms would
+    // otherwise divide out per triangle); SA adds the laminar viscosity
+    // (s_mul) and nu_tilde (s_nut); laminar: s_mul is s_mu
     double *s_mu = reinterpret_cast<double *>(s_coord + NLa);
-    double *s_mul = TURB ? s_mu + NLa : s_mu;
-    double *s_nut = s_mul + (TURB ? NLa : 0);
     double *s_e = s_mu + (VISC ? NLa : 0);
+    double *s_mul = TURB ? s_e + NLa : s_mu;
+    double *s_nut = s_mul + (TURB ? NLa : 0);
     // gradients in LDS: FP64, or rounded to FP32 in the split mode, whose
     // reconstruction reads them in FP32 (the FP64 values go to qx, qy)
     using GT = typename std::conditional<PREC == 1, float4, double4>::type;
-    GT *s_gx = reinterpret_cast<GT *>(s_mu + NLa * (TURB ? 3 : (VISC ? 2 : 0)));
+    GT *s_gx = reinterpret_cast<GT *>(s_mu + NLa * (TURB ? 4 : (VISC ? 2 : 0)));
     GT *s_gy = s_gx + NG;
     // the gradients' area factor ais of G, gathered with the state so that
     // the gradient phase reads no global memory
@@ -704,11 +704,12 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         if (i < NG) s_ais[i] = d.ais[gv];
         if (VISC) {
             double ml = sutherland_viscosity(c, p.x, p.w);
+            s_e[i] = p.w / p.x;
             if (TURB) {
                 double nu = d.nut[gv];
                 s_mul[i] = ml; s_mu[i] = ml + mu_turb(c, ml, p.x, nu); s_nut[i] = nu;
             } else {
-                s_mu[i] = ml; s_e[i] = p.w / p.x;
+                s_mu[i] = ml;
             }
         }
     }
@@ -757,12 +758,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
                          const double *bx, const double *by, const double *dxt, const double *dyt,
                          double &c1, double &c2, double &c3) {
         const double n2b3 = 2.0 / 3.0;
-        double e1, e2, e3;
-        if (TURB) {
-            e1 = p1.v[3] / p1.v[0]; e2 = p2.v[3] / p2.v[0]; e3 = p3.v[3] / p3.v[0];
-        } else {
-            e1 = s_e[tr.x]; e2 = s_e[tr.y]; e3 = s_e[tr.z];
-        }
+        const double e1 = s_e[tr.x], e2 = s_e[tr.y], e3 = s_e[tr.z];
         double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
         double eyt = e1 * by[0] + e2 * by[1] + e3 * by[2];
         double um = (p1.v[1] + p2.v[1] + p3.v[1]) / 3.0;

@@ -51,7 +51,7 @@ struct PatchArgs {
 
 // bytes of the LDS carve-up of one patch; the kernel and the planner both use it
 // flow: 1 Euler, 2 laminar (viscosity and p/rho per local vertex), 3 SA
-// (laminar and total viscosity, nu_tilde per local vertex)
+// (also the laminar viscosity and nu_tilde per local vertex)
 __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, int NE, int NB,
                                                int NGTL, int NPSL, int flow, bool split)
 {
@@ -63,7 +63,7 @@ __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, i
     // ais of G, gathered with the state
     const int pre = ((NG + 1) & ~1) * 8;
     // gradients: FP64, or FP32 in the split mode (its reconstruction reads FP32)
-    return NLa * 48 + NLa * 8 * (flow == 3 ? 3 : (visc ? 2 : 0)) + NG * (split ? 32 : 64) + pre + work +
+    return NLa * 48 + NLa * 8 * (flow == 3 ? 4 : (visc ? 2 : 0)) + NG * (split ? 32 : 64) + pre + work +
            ((index + 15) & ~15);
 }
 
