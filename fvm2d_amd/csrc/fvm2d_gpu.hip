@@ -651,7 +651,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     const int NGa = (NG + 1) & ~1;
     const int *halo = pa.halo + hdr(PH_HALO);
     const short4 *ltri = pa.ltri + hdr(PH_TRI);
-    const double *larea = pa.larea + hdr(PH_TRI);
+    const double *lqinv = pa.lqinv + hdr(PH_TRI);
     const int *gptr = pa.gptr + hdr(PH_GPTR);
     const short *gtl = pa.gtl + hdr(PH_GTL);
     const short2 *ledge = pa.ledge + hdr(PH_EDGE);
@@ -682,13 +682,13 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     // the gradient phase reads no global memory
     double *s_ais = reinterpret_cast<double *>(s_gy + NG);
     double *s_work = s_ais + NGa;
-    // work area: first the local triangles (and, with viscosity, their
-    // areas), whose terms each vertex recomputes from LDS in its ordered sum
+    // work area: first the local triangles (and, with viscosity, 0.25 over
+    // their areas), whose terms each vertex recomputes from LDS in its ordered sum
     // (same operands in the same order, so the same bits as terms computed
     // once); then the edge fluxes
     constexpr int TW = VISC ? 2 : 1;           // doubles per local triangle
     short4 *s_ltri = reinterpret_cast<short4 *>(s_work);
-    double *s_larea = s_work + NT;
+    double *s_lqinv = s_work + NT;
     const int WB = max(NT * TW * 8, (NE + 2 * NB) * 32);
     int *s_gptr = reinterpret_cast<int *>(reinterpret_cast<char *>(s_work) + WB);
     int *s_pptr = s_gptr + NG + 1;
@@ -719,7 +719,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     for (int i = t; i < NPSL; i += kBlock) s_psl[i] = pslot[i];
     for (int i = t; i < NT; i += kBlock) {
         s_ltri[i] = ltri[i];
-        if (VISC) s_larea[i] = larea[i];
+        if (VISC) s_lqinv[i] = lqinv[i];
     }
     __syncthreads();
     if (stop) return;
@@ -783,7 +783,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         }
         double tx = txx * um + txy * vm + gpr * ext;
         double ty = txy * um + tyy * vm + gpr * eyt;
-        double areat = 0.25 / s_larea[e >> 2];
+        double areat = s_lqinv[e >> 2];       // 0.25 / elarea, as the reference divides
         const int m = e & 3;
         const double bxm = bx[m], bym = by[m];
         c1 = (txx * bxm + txy * bym) * areat;
@@ -1147,7 +1147,7 @@ __device__ __forceinline__ void sa_patch_body(const Dev &d, const Consts &c, Dev
     const int NGTL = hdr(PH_NGTL), NPSL = hdr(PH_NPSL);
     const int *halo = pa.halo + hdr(PH_HALO);
     const short4 *ltri = pa.ltri + hdr(PH_TRI);
-    const double *larea = pa.larea + hdr(PH_TRI);
+    const double *lqinv = pa.lqinv + hdr(PH_TRI);
     const int *gptr = pa.gptr + hdr(PH_GPTR);
     const short *gtl = pa.gtl + hdr(PH_GTL);
     const short2 *ledge = pa.ledge + hdr(PH_EDGE);
@@ -1163,8 +1163,8 @@ __device__ __forceinline__ void sa_patch_body(const Dev &d, const Consts &c, Dev
     double2 *s_coord = reinterpret_cast<double2 *>(s_prim + NG);
     double *s_nut = reinterpret_cast<double *>(s_coord + NG);
     double *s_mul = s_nut + NGa;
-    double *s_larea = s_mul + NGa;
-    double2 *s_eds = reinterpret_cast<double2 *>(s_larea + NTa);
+    double *s_lqinv = s_mul + NGa;
+    double2 *s_eds = reinterpret_cast<double2 *>(s_lqinv + NTa);
     short4 *s_ltri = reinterpret_cast<short4 *>(s_eds + NE);
     short2 *s_ledge = reinterpret_cast<short2 *>(s_ltri + NT);
     short2 *s_lbnd = s_ledge + NE;
@@ -1181,7 +1181,7 @@ __device__ __forceinline__ void sa_patch_body(const Dev &d, const Consts &c, Dev
         s_nut[i] = d.nut[gv];
         s_mul[i] = d.mul[gv];
     }
-    for (int i = t; i < NT; i += kBlock) { s_ltri[i] = ltri[i]; s_larea[i] = larea[i]; }
+    for (int i = t; i < NT; i += kBlock) { s_ltri[i] = ltri[i]; s_lqinv[i] = lqinv[i]; }
     for (int i = t; i < NE; i += kBlock) { s_ledge[i] = ledge[i]; s_eds[i] = ledge_ds[i]; }
     for (int i = t; i < NB; i += kBlock) { s_lbnd[i] = lbnd[i]; s_lbid[i] = lbnd_id[i]; }
     for (int i = t; i <= NP; i += kBlock) { s_gptr[i] = gptr[i]; s_pptr[i] = pptr[i]; }
@@ -1204,7 +1204,7 @@ __device__ __forceinline__ void sa_patch_body(const Dev &d, const Consts &c, Dev
         const int e = s_gtl[k];
         const short4 n = s_ltri[e >> 2];
         const double2 x1 = s_coord[n.x], x2 = s_coord[n.y], x3 = s_coord[n.z];
-        const double areat = 0.25 / s_larea[e >> 2];
+        const double areat = s_lqinv[e >> 2];     // 0.25 / elarea
         const double bx[3] = {x2.y - x3.y, x3.y - x1.y, x1.y - x2.y};
         const double by[3] = {x3.x - x2.x, x1.x - x3.x, x2.x - x1.x};
         const double n1 = s_nut[n.x], n2 = s_nut[n.y], n3 = s_nut[n.z];
@@ -1877,7 +1877,6 @@ struct PatchHost {
     std::vector<short> gtl, pslot;
     std::vector<short2> ledge, lbnd;
     std::vector<double2> ledge_ds;
-    std::vector<double> larea;
     int lds_max = 0;
     int sa_lds_max = 0;
 };
@@ -2390,9 +2389,12 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     TRY(upload(x, (short4 **)&x->pa.ltri, L.ph.ltri));
     TRY(upload(x, (int **)&x->pa.gtri, L.ph.gtri));
     {
-        std::vector<double> larea(L.ph.gtri.size());
-        for (size_t i = 0; i < larea.size(); ++i) larea[i] = m->elarea[L.ph.gtri[i]];
-        TRY(upload(x, (double **)&x->pa.larea, larea));
+        // 0.25 / elarea per local triangle: the quotient the viscous and SA
+        // terms use, divided once here (IEEE division, the same bits as on
+        // the device) instead of once per vertex and triangle
+        std::vector<double> lqinv(L.ph.gtri.size());
+        for (size_t i = 0; i < lqinv.size(); ++i) lqinv[i] = 0.25 / m->elarea[L.ph.gtri[i]];
+        TRY(upload(x, (double **)&x->pa.lqinv, lqinv));
         std::vector<double2> lds(L.ph.ledge_id.size());
         for (size_t i = 0; i < lds.size(); ++i) lds[i] = L.eds[L.ph.ledge_id[i]];
         TRY(upload(x, (double2 **)&x->pa.ledge_ds, lds));

@@ -34,7 +34,7 @@ struct PatchArgs {
     const int *halo;        // new vertex ids of H1 then H2
     const short4 *ltri;     // local nodes of each local triangle
     const int *gtri;        // its triangle number
-    const double *larea;    // its area (elarea)
+    const double *lqinv;    // 0.25 / its area (elarea)
     const int *gptr;        // per G-vertex: range in gtl (patch-relative)
     const short *gtl;       // 4 * local triangle + position of the vertex in it
     const short2 *ledge;    // local end points of each local edge

@@ -4,13 +4,16 @@ inputs.  Every call goes through the C-ABI (libfvm2d_gpu.so).
 
 Tolerances (the FP64 bar: residual within 1e-11 relative, CL/CD and the
 residual history within 1e-8):
-  * Roe and HLLC, Euler: no transcendental function on the path -> the
-    residual, the states and (iter, iresi, fresi) are bit-identical; fres,
-    cl and cd come from tree reductions on the GPU: 1e-12 relative;
-  * KFVS (erf, exp), laminar (pow in Sutherland), SA (pow) and the vortex
-    farfield (atan2, sin, cos, pow): device libm may differ from glibc in
-    the last bit: residual 1e-11 relative (max norm), history and states
-    1e-8.
+  * Roe and HLLC without the vortex farfield, Euler, laminar and SA: the
+    residual, the states (and nu_tilde) and (iter, iresi, fresi) are
+    bit-identical; fres, cl and cd come from tree reductions on the GPU:
+    1e-12 relative.  The only powers on these paths are Sutherland's
+    x**1.5, which the GPU rounds correctly (glibc's pow gives the same
+    bits on these inputs), and SA's pow(., 1/6), the device libm's, which
+    also agrees with glibc on these inputs;
+  * KFVS (erf, exp) and the vortex farfield (atan2, sin, cos, pow): device
+    libm may differ from glibc in the last bit: residual 1e-11 relative
+    (max norm), history and states 1e-8.
 """
 import numpy as np
 import pytest
@@ -24,8 +27,7 @@ from tests.helpers import case_kwargs, fixture, grid, rel_err
 pytestmark = pytest.mark.gpu
 
 BITWISE = {n for n, c in CASES.items()
-           if c["deck"]["FlowType"] == 1 and c["deck"]["Flux"] in (1, 3)
-           and c["deck"]["FarBC"][0] == 0}
+           if c["deck"]["Flux"] in (1, 3) and c["deck"]["FarBC"][0] == 0}
 
 
 def solver_for(name, **over):
@@ -87,7 +89,7 @@ def test_iterations_match_reference(name):
         assert err <= tol, (col, err)
     assert rel_err(st["prim"], fx["primN"]) <= (0 if bit else 1e-8)
     if CASES[name]["deck"]["FlowType"] == 3:
-        assert rel_err(st["nut"], fx["nutN"]) <= 1e-8
+        assert rel_err(st["nut"], fx["nutN"]) <= (0 if bit else 1e-8)
 
 
 def test_bitwise_run_to_run():
