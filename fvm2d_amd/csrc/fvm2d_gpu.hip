@@ -45,9 +45,6 @@ constexpr int kBlock = 256;
 // waves per SIMD the stage kernel is compiled for (one workgroup of
 // kBlock threads puts one wave on each SIMD of a CU)
 constexpr int kPatchWavesEuler = 4, kPatchWavesVisc = 3, kPatchWavesSplit = 4;
-// patch vertices whose viscous stress terms groups of three lanes can take
-// (21 groups a wavefront)
-constexpr int kViscGroups = 21 * (kBlock / 64);
 constexpr int kHistCap = 4096;      // history rows per fvm_gpu_iterate chunk
 constexpr int kGraphIters = 8;      // iterations captured per graph
 
@@ -644,7 +641,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     auto hdr = [&](int k) { return uniform_at(pa.hdr, hb + k); };
     const int v0 = hdr(PH_V0), NP = hdr(PH_NP), NL = hdr(PH_NL), NG = hdr(PH_NG);
     const int NT = hdr(PH_NTRI), NE = hdr(PH_NEDGE), NB = hdr(PH_NBND);
-    const int NGTL = hdr(PH_NGTL), NPSL = hdr(PH_NPSL);
+    const int NGTL = hdr(PH_NGTL), NPSL = hdr(PH_NPSL), NTP = VISC ? hdr(PH_NTP) : 0;
     extern __shared__ double4 smem4[];
     double *smem = reinterpret_cast<double *>(smem4);
     const int NLa = (NL + 1) & ~1;               // keeps every LDS array 16-byte aligned
@@ -678,9 +675,12 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     using GT = typename std::conditional<PREC == 1, float4, double4>::type;
     GT *s_gx = reinterpret_cast<GT *>(s_mu + NLa * (TURB ? 4 : (VISC ? 2 : 0)));
     GT *s_gy = s_gx + NG;
-    // the gradients' area factor ais of G, gathered with the state so that
-    // the gradient phase reads no global memory
-    double *s_ais = reinterpret_cast<double *>(s_gy + NG);
+    // (with viscosity the gradients' bytes, at least 40 per triangle touching
+    // P, first stage the stress terms); then the gradients' area factor ais
+    // of G, gathered with the state so that the gradient phase reads no
+    // global memory
+    const int GRB = ((VISC && NTP * 40 > NG * (int)(2 * sizeof(GT))) ? NTP * 40 : NG * (int)(2 * sizeof(GT)));
+    double *s_ais = reinterpret_cast<double *>(reinterpret_cast<char *>(s_gx) + ((GRB + 15) & ~15));
     double *s_work = s_ais + NGa;
     // work area: first the local triangles (and, with viscosity, 0.25 over
     // their areas), whose terms each vertex recomputes from LDS in its ordered sum
@@ -749,67 +749,76 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         for (int kb = kBlock - 1 - t; kb < NB; kb += kBlock) bnd_flux(kb);
 
     // 2. gradients of G (src/green_gauss.f): each vertex sums its triangles'
-    // terms in ascending triangle order; with viscosity a vertex of P also
-    // sums the triangles' Galerkin stress terms into its residual
-    // (src/green_gauss.f:132-194), evaluated here from LDS.
+    // terms in ascending triangle order, each term recomputed from LDS.
     //
-    // The stress term of triangle e (position m of the vertex) for one vertex
-    auto visc_term = [&](const int e, const short4 tr, const V4 &p1, const V4 &p2, const V4 &p3,
-                         const double *bx, const double *by, const double *dxt, const double *dyt,
-                         double &c1, double &c2, double &c3) {
-        const double n2b3 = 2.0 / 3.0;
-        const double e1 = s_e[tr.x], e2 = s_e[tr.y], e3 = s_e[tr.z];
-        double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
-        double eyt = e1 * by[0] + e2 * by[1] + e3 * by[2];
-        double um = (p1.v[1] + p2.v[1] + p3.v[1]) / 3.0;
-        double vm = (p1.v[2] + p2.v[2] + p3.v[2]) / 3.0;
-        double mut = (s_mu[tr.x] + s_mu[tr.y] + s_mu[tr.z]) / 3.0;
-        double txx = n2b3 * mut * (2.0 * dxt[1] - dyt[2]);
-        double txy = mut * (dyt[1] + dxt[2]);
-        double tyy = n2b3 * mut * (2.0 * dyt[2] - dxt[1]);
-        double gpr;
-        if (TURB) {
-            double mult = (s_mul[tr.x] + s_mul[tr.y] + s_mul[tr.z]) / 3.0;
-            double prim5t = (s_nut[tr.x] + s_nut[tr.y] + s_nut[tr.z]) / 3.0;
-            double rhot = (p1.v[0] + p2.v[0] + p3.v[0]) / 3.0;
-            double chi = prim5t * rhot / mult;
-            double chi3 = pow3(chi);
-            double fv1 = chi3 / (chi3 + c.Cv11);
-            gpr = c.GAMMA * (mult / c.prandtl + rhot * prim5t * fv1 / c.prandtl_turb) / c.GAMMA1;
-        } else {
-            // nu_tilde is zero, so the eddy term is +0 and the laminar
-            // viscosity is the total one: the same value, fewer operations
-            gpr = c.GAMMA * (mut / c.prandtl) / c.GAMMA1;
-        }
-        double tx = txx * um + txy * vm + gpr * ext;
-        double ty = txy * um + tyy * vm + gpr * eyt;
-        double areat = s_lqinv[e >> 2];       // 0.25 / elarea, as the reference divides
-        const int m = e & 3;
-        const double bxm = bx[m], bym = by[m];
-        c1 = (txx * bxm + txy * bym) * areat;
-        c2 = (txy * bxm + tyy * bym) * areat;
-        c3 = (tx * bxm + ty * bym) * areat;
-    };
-    // With viscosity and a patch of at most kViscGroups vertices, a vertex's
-    // stress terms are evaluated by a group of three lanes (below); larger
-    // patches add them in the gradient loop.
-    const bool grouped = VISC && NP <= kViscGroups;
+    // With viscosity, first the Galerkin stress terms (src/green_gauss.f:
+    // 132-194): (a) each triangle touching P (local numbers below NTP, as the
+    // triangles are numbered while P's vertices are scanned) evaluates its
+    // stresses and heat flux once and stages them in the gradients' LDS;
+    // (b) each vertex of P adds, in ascending triangle order, each triangle's
+    // stresses against its own side (bx[m], by[m]) times 0.25/area: the
+    // operations of the per-triangle loop of the reference, term for term.
     double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
+    if (VISC) {
+        double *s_vt = reinterpret_cast<double *>(s_gx);          // 5 per triangle
+        for (int k = t; k < NTP; k += kBlock) {
+            const short4 tr = s_ltri[k];
+            V4 p1, p2, p3;
+            double bx[3], by[3], dxt[4], dyt[4];
+            tri_gg(s_coord, s_prim, tr, p1, p2, p3, bx, by, dxt, dyt);
+            const double n2b3 = 2.0 / 3.0;
+            const double e1 = s_e[tr.x], e2 = s_e[tr.y], e3 = s_e[tr.z];
+            double ext = e1 * bx[0] + e2 * bx[1] + e3 * bx[2];
+            double eyt = e1 * by[0] + e2 * by[1] + e3 * by[2];
+            double um = (p1.v[1] + p2.v[1] + p3.v[1]) / 3.0;
+            double vm = (p1.v[2] + p2.v[2] + p3.v[2]) / 3.0;
+            double mut = (s_mu[tr.x] + s_mu[tr.y] + s_mu[tr.z]) / 3.0;
+            double txx = n2b3 * mut * (2.0 * dxt[1] - dyt[2]);
+            double txy = mut * (dyt[1] + dxt[2]);
+            double tyy = n2b3 * mut * (2.0 * dyt[2] - dxt[1]);
+            double gpr;
+            if (TURB) {
+                double mult = (s_mul[tr.x] + s_mul[tr.y] + s_mul[tr.z]) / 3.0;
+                double prim5t = (s_nut[tr.x] + s_nut[tr.y] + s_nut[tr.z]) / 3.0;
+                double rhot = (p1.v[0] + p2.v[0] + p3.v[0]) / 3.0;
+                double chi = prim5t * rhot / mult;
+                double chi3 = pow3(chi);
+                double fv1 = chi3 / (chi3 + c.Cv11);
+                gpr = c.GAMMA * (mult / c.prandtl + rhot * prim5t * fv1 / c.prandtl_turb) / c.GAMMA1;
+            } else {
+                // nu_tilde is zero, so the eddy term is +0 and the laminar
+                // viscosity is the total one: the same value, fewer operations
+                gpr = c.GAMMA * (mut / c.prandtl) / c.GAMMA1;
+            }
+            double *o = s_vt + 5 * k;
+            o[0] = txx; o[1] = txy; o[2] = tyy;
+            o[3] = txx * um + txy * vm + gpr * ext;
+            o[4] = txy * um + tyy * vm + gpr * eyt;
+        }
+        __syncthreads();
+        for (int g = t; g < NP; g += kBlock) {
+            const int k1 = s_gptr[g + 1];
+            for (int k = s_gptr[g]; k < k1; ++k) {
+                const int e = s_gtl[k], tri = e >> 2, m = e & 3;
+                const short4 tr = s_ltri[tri];
+                // the side opposite corner m: bx[m] = y(m+1) - y(m+2),
+                // by[m] = x(m+2) - x(m+1)
+                const int ia = m == 0 ? tr.y : (m == 1 ? tr.z : tr.x);
+                const int ib = m == 0 ? tr.z : (m == 1 ? tr.x : tr.y);
+                const double2 xa = s_coord[ia], xb = s_coord[ib];
+                const double bxm = xa.y - xb.y, bym = xb.x - xa.x;
+                const double areat = s_lqinv[tri];       // 0.25 / elarea
+                const double *o = s_vt + 5 * tri;
+                rv1 = rv1 + (o[0] * bxm + o[1] * bym) * areat;
+                rv2 = rv2 + (o[1] * bxm + o[2] * bym) * areat;
+                rv3 = rv3 + (o[3] * bxm + o[4] * bym) * areat;
+            }
+        }
+        __syncthreads();                 // the gradients overwrite the staged terms
+    }
     for (int g = t; g < NG; g += kBlock) {
         const double ais = s_ais[g];
         double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
-        auto add_tri = [&](const int e, const short4 tr, const V4 &p1, const V4 &p2, const V4 &p3,
-                           const double *bx, const double *by, const double *dxt, const double *dyt) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) { gx[i] = gx[i] + dxt[i]; gy[i] = gy[i] + dyt[i]; }
-            if (VISC && !grouped && g < NP) {
-                double c1, c2, c3;
-                visc_term(e, tr, p1, p2, p3, bx, by, dxt, dyt, c1, c2, c3);
-                rv1 = rv1 + c1;
-                rv2 = rv2 + c2;
-                rv3 = rv3 + c3;
-            }
-        };
         const int k1 = s_gptr[g + 1];
         for (int k = s_gptr[g]; k < k1; ++k) {
             const int e = s_gtl[k];
@@ -817,7 +826,8 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             V4 p1, p2, p3;
             double bx[3], by[3], dxt[4], dyt[4];
             tri_gg(s_coord, s_prim, tr, p1, p2, p3, bx, by, dxt, dyt);
-            add_tri(e, tr, p1, p2, p3, bx, by, dxt, dyt);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) { gx[i] = gx[i] + dxt[i]; gy[i] = gy[i] + dyt[i]; }
         }
         double4 qx = make_double4(gx[0] * ais, gx[1] * ais, gx[2] * ais, gx[3] * ais);
         double4 qy = make_double4(gy[0] * ais, gy[1] * ais, gy[2] * ais, gy[3] * ais);
@@ -830,39 +840,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         }
         if (MODE != 0 && g < NP) { d.qx[v0 + g] = qx; d.qy[v0 + g] = qy; }
     }
-    // Stress terms in groups of three lanes: lane j of vertex v's group
-    // evaluates its triangles j, j + 3, ...; the running sums go round the
-    // group (lane 0, 1, 2, 0, ...), each lane adding its term in triangle
-    // order, so they are the sums of the loop above, bit for bit.  21 groups
-    // a wavefront, filled from the last wavefront, which the gradients leave
-    // idle; the sums end in the group's first lane, which then updates v.
-    int tl = t;                                   // local vertex this thread updates
-    if (grouped) {
-        const int w = t >> 6, lane = t & 63, gs = lane / 3, j = lane - 3 * gs;
-        const int v = gs < 21 ? 21 * (kBlock / 64 - 1 - w) + gs : NP;
-        const bool act = v < NP;
-        const int kb0 = act ? s_gptr[v] : 0, cnt = act ? s_gptr[v + 1] - kb0 : 0;
-        const int prev = 3 * gs + (j == 0 ? 2 : j - 1);   // lane before this one in the group
-        for (int q = 0; __any(3 * q < cnt); ++q) {
-            const int k = 3 * q + j;
-            const bool have = k < cnt;
-            double c1 = 0.0, c2 = 0.0, c3 = 0.0;
-            if (have) {
-                const int e = s_gtl[kb0 + k];
-                const short4 tr = s_ltri[e >> 2];
-                V4 p1, p2, p3;
-                double bx[3], by[3], dxt[4], dyt[4];
-                tri_gg(s_coord, s_prim, tr, p1, p2, p3, bx, by, dxt, dyt);
-                visc_term(e, tr, p1, p2, p3, bx, by, dxt, dyt, c1, c2, c3);
-            }
-#pragma unroll
-            for (int step = 0; step < 3; ++step) {
-                if (j == step && have) { rv1 = rv1 + c1; rv2 = rv2 + c2; rv3 = rv3 + c3; }
-                rv1 = __shfl(rv1, prev); rv2 = __shfl(rv2, prev); rv3 = __shfl(rv3, prev);
-            }
-        }
-        tl = (act && j == 0 && gs < 21) ? v : NP;
-    }
+    const int tl = t;                             // local vertex this thread updates
     __syncthreads();
 
     // 3. fluxes of the edges and boundary edges touching P
@@ -1935,6 +1913,7 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
                 ph.gtl.push_back((short)(4 * tloc[t] + m));
             }
             ph.gptr.push_back((int)ph.gtl.size() - h[PH_GTL]);
+            if (g == NP - 1) h[PH_NTP] = (int)tl.size();     // P's triangles come first
         }
         h[PH_NTRI] = (int)tl.size();
         // edges and boundary edges touching P, each once
@@ -1986,8 +1965,8 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
         if (NL > 32767 || 4 * (int)tl.size() > 32767 || 2 * (NE + 2 * NB) > 32767) return FVM_E_SIZE;
         h[PH_NGTL] = (int)ph.gtl.size() - h[PH_GTL];
         h[PH_NPSL] = (int)ph.pslot.size() - h[PH_PSL];
-        const int lds = patch_lds_bytes(NP, NL, NG, (int)tl.size(), NE, NB, h[PH_NGTL], h[PH_NPSL], iflow,
-                                        split);
+        const int lds = patch_lds_bytes(NP, NL, NG, (int)tl.size(), NE, NB, h[PH_NGTL], h[PH_NPSL],
+                                        h[PH_NTP], iflow, split);
         ph.lds_max = std::max(ph.lds_max, lds);
         ph.sa_lds_max = std::max(ph.sa_lds_max, sa_lds_bytes(NP, NG, (int)tl.size(), NE, NB, h[PH_NGTL],
                                                                h[PH_NPSL]));

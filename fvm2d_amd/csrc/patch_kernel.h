@@ -26,7 +26,8 @@ namespace fvm {
 constexpr int kPatchHdr = 24;
 enum {
     PH_V0 = 0, PH_NP, PH_NL, PH_NG, PH_HALO, PH_NTRI, PH_TRI, PH_GPTR, PH_GTL,
-    PH_NEDGE, PH_EDGE, PH_NBND, PH_BND, PH_PPTR, PH_PSL, PH_NGTL, PH_NPSL
+    PH_NEDGE, PH_EDGE, PH_NBND, PH_BND, PH_PPTR, PH_PSL, PH_NGTL, PH_NPSL,
+    PH_NTP                  // local triangles touching P (numbered first)
 };
 
 struct PatchArgs {
@@ -53,7 +54,7 @@ struct PatchArgs {
 // flow: 1 Euler, 2 laminar (viscosity and p/rho per local vertex), 3 SA
 // (also the laminar viscosity and nu_tilde per local vertex)
 __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, int NE, int NB,
-                                               int NGTL, int NPSL, int flow, bool split)
+                                               int NGTL, int NPSL, int NTP, int flow, bool split)
 {
     const bool visc = flow != 1;
     const int NLa = (NL + 1) & ~1;
@@ -62,8 +63,12 @@ __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, i
     const int index = 4 * (NG + 1) + 4 * (NP + 1) + 2 * (NGTL + NPSL);
     // ais of G, gathered with the state
     const int pre = ((NG + 1) & ~1) * 8;
-    // gradients: FP64, or FP32 in the split mode (its reconstruction reads FP32)
-    return NLa * 48 + NLa * 8 * (flow == 3 ? 4 : (visc ? 2 : 0)) + NG * (split ? 32 : 64) + pre + work +
+    // gradients: FP64, or FP32 in the split mode (its reconstruction reads
+    // FP32); with viscosity the same bytes first stage 5 doubles of stress
+    // terms per triangle touching P
+    const int grad = NG * (split ? 32 : 64);
+    const int grad_region = (((visc && NTP * 40 > grad) ? NTP * 40 : grad) + 15) & ~15;
+    return NLa * 48 + NLa * 8 * (flow == 3 ? 4 : (visc ? 2 : 0)) + grad_region + pre + work +
            ((index + 15) & ~15);
 }
 
