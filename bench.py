@@ -10,7 +10,9 @@ The grid and state are resident in HBM before timing starts.  Prints one
 JSON line with the throughput in vertex-iterations/s, the roofline of the
 dominant kernel (algorithmic bytes per launch / its HIP-event duration on
 the solver's stream) and the CPU baseline (the oracle, single core, on a
-bounded sample of the same workload).
+bounded sample of the same workload); beside them split precision, batches
+of solves, and the serving loop's p50/p99 solve latency on a bounded stream
+of requests.
 """
 from __future__ import annotations
 
@@ -158,6 +160,44 @@ def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
     return out
 
 
+def serving_leg(n: int = 32, seed: int = 11, max_iters: int = 300) -> dict:
+    """Solve latency of the serving loop (fvm2d_amd/serving.py) on a bounded
+    stream: n seeded requests of 600-6000 vertices (NACA 4-digit airfoils,
+    70/30 Euler/SA-RANS), each to a 1e-6 residual drop or max_iters
+    iterations, in split precision.  Offered at 400 requests/s (faster than
+    the loop serves them), the run gives the saturation rate in solves/s; then the same requests arrive as Poisson
+    processes at 50 % and 90 % of that rate.  A latency runs from a request's
+    arrival to its result: host meshing and preprocessing, admission, the
+    solve."""
+    from fvm2d_amd import serving
+    cfg = serving.ServeConfig(max_iters=max_iters, step_vertex_iters=2e5, minres=1e-6,
+                              size_classes=(2500,))
+
+    def run(rate: float) -> dict:
+        reqs = serving.poisson_arrivals(serving.workload(n, seed, np_min=600, np_max=6000),
+                                        rate, seed)
+        srv = serving.Server(cfg)
+        res = srv.run(reqs, timeout=120.0)
+        s = serving.summary(res, srv.wall)
+        out = {"offered_per_s": round(rate, 2), "solves": s["solves"], "wall_s": s["wall_s"],
+               "solves_per_s": round(s["solves_per_s"], 2),
+               "vertex_iters_per_s": round(s["vertex_iters_per_s"], 1),
+               "p50_latency_s": round(s["p50_latency_s"], 4),
+               "p99_latency_s": round(s["p99_latency_s"], 4),
+               "converged": s["converged"], "capped": s["capped"], "failed": s["failed"],
+               "peak_in_flight": srv.peak_in_flight}
+        if srv.errors:
+            out["errors"] = srv.errors
+        return out
+
+    run(400.0)                          # first pass: one-time costs of the loop
+    sat = run(400.0)
+    rate = sat["solves_per_s"]
+    return {"workload": f"{n} requests, NACA 4-digit O-grids of 600-6000 vertices, 70/30 "
+                        f"Euler/SA-RANS, split precision, residual drop 1e-6 or {max_iters} iterations",
+            "saturation": sat, "load_50": run(0.5 * rate), "load_90": run(0.9 * rate)}
+
+
 def config_legs(iters_lam: int, iters_rans: int) -> dict:
     """Split precision against FP64 on the viscous workloads.
 
@@ -235,6 +275,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the split-precision and batched legs")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency leg")
     ap.add_argument("--variant", type=int, default=None, help="kernel layout 1 or 2 (default: tuned)")
     ap.add_argument("--vb", type=int, default=64, help="vertices per workgroup (variant 2)")
     ap.add_argument("--tune", action="store_true",
@@ -338,6 +379,7 @@ def main() -> int:
                  "note": "same workload and step count with FP32 reconstruction and flux; "
                          "dcl, dcd against this FP64 run"}
     batched = None if a.no_batch else batch_legs(g, s.geo, kw, value, a.warmup)
+    served = None if a.no_serving else serving_leg()
 
     cpu = None if a.no_cpu else cpu_baseline(g, {k: v for k, v in kw.items() if k != "patch_size"})
     line = {
@@ -350,6 +392,7 @@ def main() -> int:
                                % {1: "Roe", 2: "KFVS", 3: "HLLC"}[a.flux],
                    "np": npt, "nt": nt, "ne": ne, "precision": "fp64", "launch": launch},
         "roofline": roofline, "cpu_baseline": cpu, "split_precision": split, "batched": batched,
+        "serving": served,
         "cl": float(hist[-1, 4]), "cd": float(hist[-1, 5]),
     }
     if cpu:
