@@ -197,17 +197,18 @@ __global__ void __launch_bounds__(kBlock) k_flux(Dev d, Consts c, DevState *S)
         V4 qx1 = ld4(d.qx, e.x), qy1 = ld4(d.qy, e.x), qx2 = ld4(d.qx, e.y), qy2 = ld4(d.qy, e.y);
         V4 pl, pr, f;
         recon<LIM>(c, x2.x - x1.x, x2.y - x1.y, p1, p2, qx1, qy1, qx2, qy2, pl, pr);
+        const EdgeFrame<double> fr = edge_frame(ds.x, ds.y);
         if (FLUX == 1) {
-            if (!roe_flux(c, ds.x, ds.y, pl, pr, f)) {
+            if (!roe_flux(c, fr, pl, pr, f)) {
                 atomicMin(&S->err_index, i + 1);       // position in the live-edge loop
                 S->err_code = FVM_E_NEG_SOUND;
             }
         } else if (FLUX == 2) {
-            kfvs_flux(c, ds.x, ds.y, pl, pr, f);
+            kfvs_flux(c, fr, pl, pr, f);
         } else if (FLUX == 3) {
-            hllc_flux<0>(c, ds.x, ds.y, pl, pr, f);
+            hllc_flux<0>(c, fr, pl, pr, f);
         } else {
-            hllc_flux<1>(c, ds.x, ds.y, pl, pr, f);
+            hllc_flux<1>(c, fr, pl, pr, f);
         }
         st4(d.eflux, i, f);
         return;
@@ -653,7 +654,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     const short *gtl = pa.gtl + hdr(PH_GTL);
     const short2 *ledge = pa.ledge + hdr(PH_EDGE);
     const int *ledge_id = pa.ledge_id + hdr(PH_EDGE);
-    const double2 *ledge_ds = pa.ledge_ds + hdr(PH_EDGE);
+    const double4 *ledge_fr = pa.ledge_fr + hdr(PH_EDGE);
     const short2 *lbnd = pa.lbnd + hdr(PH_BND);
     const int *lbnd_id = pa.lbnd_id + hdr(PH_BND);
     const int *pptr = pa.pptr + hdr(PH_PPTR);
@@ -847,7 +848,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     int bad = INT_MAX;
     for (int k = t; k < NE; k += kBlock) {
         short2 e = ledge[k];
-        double2 ds = ledge_ds[k];
+        const double4 fr = ledge_fr[k];
         double2 x1 = s_coord[e.x], x2 = s_coord[e.y];
         double4 a1 = s_prim[e.x], a2 = s_prim[e.y];
         const GT g1 = s_gx[e.x], h1 = s_gy[e.x], g2 = s_gx[e.y], h2 = s_gy[e.y];
@@ -859,14 +860,15 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             V4 qx2{{g2.x, g2.y, g2.z, g2.w}}, qy2{{h2.x, h2.y, h2.z, h2.w}};
             V4 pl, pr;
             recon<LIM>(c, x2.x - x1.x, x2.y - x1.y, p1, p2, qx1, qy1, qx2, qy2, pl, pr);
-            if (FLUX == 1) ok = roe_flux(c, ds.x, ds.y, pl, pr, f);
-            else if (FLUX == 2) kfvs_flux(c, ds.x, ds.y, pl, pr, f);
-            else if (FLUX == 3) hllc_flux<0>(c, ds.x, ds.y, pl, pr, f);
-            else hllc_flux<1>(c, ds.x, ds.y, pl, pr, f);
+            const EdgeFrame<double> efr{fr.x, fr.y, fr.z};
+            if (FLUX == 1) ok = roe_flux(c, efr, pl, pr, f);
+            else if (FLUX == 2) kfvs_flux(c, efr, pl, pr, f);
+            else if (FLUX == 3) hllc_flux<0>(c, efr, pl, pr, f);
+            else hllc_flux<1>(c, efr, pl, pr, f);
         } else {
             // split precision: reconstruction and flux in FP32 on FP64 inputs
-            // (edge vector from FP64 coordinates); the flux returns to FP64
-            // for the accumulation
+            // (edge vector from FP64 coordinates, edge frame rounded from
+            // FP64); the flux returns to FP64 for the accumulation
             V4f pl, pr, ff;
             const V4f qx1{{(float)g1.x, (float)g1.y, (float)g1.z, (float)g1.w}};
             const V4f qy1{{(float)h1.x, (float)h1.y, (float)h1.z, (float)h1.w}};
@@ -874,11 +876,11 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             const V4f qy2{{(float)h2.x, (float)h2.y, (float)h2.z, (float)h2.w}};
             const float dxf = (float)(x2.x - x1.x), dyf = (float)(x2.y - x1.y);
             recon<LIM, float>(c, dxf, dyf, to_f(p1), to_f(p2), qx1, qy1, qx2, qy2, pl, pr);
-            const float sxf = (float)ds.x, syf = (float)ds.y;
-            if (FLUX == 1) ok = roe_flux<float>(c, sxf, syf, pl, pr, ff);
-            else if (FLUX == 2) kfvs_flux<float>(c, sxf, syf, pl, pr, ff);
-            else if (FLUX == 3) hllc_flux<0, float>(c, sxf, syf, pl, pr, ff);
-            else hllc_flux<1, float>(c, sxf, syf, pl, pr, ff);
+            const EdgeFrame<float> efr{(float)fr.x, (float)fr.y, (float)fr.z};
+            if (FLUX == 1) ok = roe_flux<float>(c, efr, pl, pr, ff);
+            else if (FLUX == 2) kfvs_flux<float>(c, efr, pl, pr, ff);
+            else if (FLUX == 3) hllc_flux<0, float>(c, efr, pl, pr, ff);
+            else hllc_flux<1, float>(c, efr, pl, pr, ff);
             f = to_d(ff);
         }
         if (!ok) bad = min(bad, ledge_id[k] + 1);
@@ -2243,9 +2245,9 @@ long long fvm_gpu_footprint_estimate(int np, int nt, int netot, int iflow, int p
     b += T * (12 + 16 + 8);                // v2t, tri, elarea
     b += E * (8 + 16 + 32 + 8);            // enode, eds, eflux, v2e
     // patch tables: local triangles (shorts, number, area), Green-Gauss and
-    // residual term lists, local edges (ends, number, normal), halo,
+    // residual term lists, local edges (ends, number, normal, frame), halo,
     // headers; bounded for patches down to 24 vertices
-    b += P * 640;
+    b += P * 800;
     b += 6LL * 8 * kHistCap + 64 * 1024;
     (void)iflow;
     return b;
@@ -2377,6 +2379,14 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
         std::vector<double2> lds(L.ph.ledge_id.size());
         for (size_t i = 0; i < lds.size(); ++i) lds[i] = L.eds[L.ph.ledge_id[i]];
         TRY(upload(x, (double2 **)&x->pa.ledge_ds, lds));
+        // the frame of each local edge's ds (length, then the two IEEE
+        // quotients: the fluxes' own first operations, done once here)
+        std::vector<double4> lfr(lds.size());
+        for (size_t i = 0; i < lfr.size(); ++i) {
+            const double length = std::sqrt(lds[i].x * lds[i].x + lds[i].y * lds[i].y);
+            lfr[i] = make_double4(lds[i].x / length, lds[i].y / length, length, 0.0);
+        }
+        TRY(upload(x, (double4 **)&x->pa.ledge_fr, lfr));
     }
     TRY(upload(x, (int **)&x->pa.gptr, L.ph.gptr));
     TRY(upload(x, (short **)&x->pa.gtl, L.ph.gtl));

@@ -68,6 +68,20 @@ __device__ __forceinline__ float fexp(float a) { return __expf(a); }
 __device__ __forceinline__ double ferf(double a) { return erf(a); }
 __device__ __forceinline__ float ferf(float a) { return erff(a); }
 
+// The frame of a dual face ds = (dsx, dsy): its length and unit normal
+// (ct, st), the first lines of src/roe_flux.f:18-20, src/kfvs_flux.f:16-18
+// and src/hllc_flux.f:18-20.  It depends on the mesh only, so the stage
+// kernel reads it from a per-edge table made once (the same IEEE square root
+// and divisions, so the same bits).
+template <class T> struct EdgeFrame { T ct, st, len; };
+
+template <class T>
+__device__ __forceinline__ EdgeFrame<T> edge_frame(T dsx, T dsy)
+{
+    const T length = fsqrt(dsx * dsx + dsy * dsy);
+    return EdgeFrame<T>{fdiv(dsx, length), fdiv(dsy, length), length};
+}
+
 // van Albada limiter, src/recon.f:55-71
 template <class T>
 __device__ __forceinline__ T limit_albada(T ul, T ur)
@@ -104,12 +118,11 @@ __device__ __forceinline__ void recon(const Consts &c, T dx, T dy, const VT<T> &
 
 // Roe flux without entropy fix, src/roe_flux.f:18-112.  false: a^2 <= 0.
 template <class T = double>
-__device__ __forceinline__ bool roe_flux(const Consts &c, T dsx, T dsy, const VT<T> &L,
+__device__ __forceinline__ bool roe_flux(const Consts &c, const EdgeFrame<T> &fr, const VT<T> &L,
                                          const VT<T> &R, VT<T> &f)
 {
     const T G = T(c.GAMMA), G1 = T(c.GAMMA1), h = T(0.5);
-    T length = fsqrt(dsx * dsx + dsy * dsy);
-    T ct = fdiv(dsx, length), st = fdiv(dsy, length);
+    const T length = fr.len, ct = fr.ct, st = fr.st;
     T rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
     T al2 = fdiv(G * pl, rl), ql2 = ul * ul + vl * vl, hl = fdiv(al2, G1) + h * ql2;
     T rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
@@ -149,12 +162,11 @@ __device__ __forceinline__ bool roe_flux(const Consts &c, T dsx, T dsy, const VT
 
 // kinetic flux-vector splitting, src/kfvs_flux.f:16-78
 template <class T = double>
-__device__ __forceinline__ void kfvs_flux(const Consts &c, T dsx, T dsy, const VT<T> &L,
+__device__ __forceinline__ void kfvs_flux(const Consts &c, const EdgeFrame<T> &fr, const VT<T> &L,
                                           const VT<T> &R, VT<T> &f)
 {
     const T G1 = T(c.GAMMA1), h = T(0.5), PI = T(c.M_PI_);
-    T length = fsqrt(dsx * dsx + dsy * dsy);
-    T ct = fdiv(dsx, length), st = fdiv(dsy, length);
+    const T length = fr.len, ct = fr.ct, st = fr.st;
     T rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
     T el = fdiv(pl, G1) + h * rl * (ul * ul + vl * vl);
     T rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
@@ -180,12 +192,11 @@ __device__ __forceinline__ void kfvs_flux(const Consts &c, T dsx, T dsy, const V
 // HLLC (Toro ch. 10), src/hllc_flux.f:18-154.  The reference adds the
 // edge-frame flux to the x-y momentum residual; ROT=1 rotates it back.
 template <int ROT, class T = double>
-__device__ __forceinline__ void hllc_flux(const Consts &c, T dsx, T dsy, const VT<T> &L,
+__device__ __forceinline__ void hllc_flux(const Consts &c, const EdgeFrame<T> &fr, const VT<T> &L,
                                           const VT<T> &R, VT<T> &f)
 {
     const T gamma = T(c.GAMMA), G1 = T(c.GAMMA1), h = T(0.5), one = T(1), two = T(2);
-    T length = sqrt(dsx * dsx + dsy * dsy);
-    T ct = dsx / length, st = dsy / length;
+    const T length = fr.len, ct = fr.ct, st = fr.st;
     T rl = L.v[0], ul = L.v[1] * ct + L.v[2] * st, vl = -L.v[1] * st + L.v[2] * ct, pl = L.v[3];
     T al = sqrt(gamma * pl / rl), El = pl / G1 + h * rl * (ul * ul + vl * vl);
     T rr = R.v[0], ur = R.v[1] * ct + R.v[2] * st, vr = -R.v[1] * st + R.v[2] * ct, pr = R.v[3];

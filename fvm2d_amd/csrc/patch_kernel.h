@@ -41,6 +41,7 @@ struct PatchArgs {
     const short2 *ledge;    // local end points of each local edge
     const int *ledge_id;    // its position in the live-edge loop
     const double2 *ledge_ds;  // its dual-face normal (ds)
+    const double4 *ledge_fr;  // the frame of ds: ct, st, |ds| (and a pad)
     const short2 *lbnd;     // local end points of each local boundary edge
     const int *lbnd_id;     // its index among the boundary edges (solid first)
     const int *pptr;        // per P-vertex: range in pslot (patch-relative)
