@@ -49,8 +49,12 @@ constexpr int kHistCap = 4096;      // history rows per fvm_gpu_iterate chunk
 constexpr int kGraphIters = 8;      // iterations captured per graph
 
 struct DevState {
-    int iter, done, status, err_code, err_index, nhist, iresi, pad;
+    int iter, done, status, err_code, err_index, nhist, iresi;
+    int err_stage;              // stage kernel (1-3; 4 the SA step) that raised err_code
     double fres, fres1, fresi, cl, cd;
+    // first-stage errors, kept apart from err_code until the second stage
+    // (the monitors of the previous iteration may run beside the first stage)
+    int err1_code, err1_index;
 };
 
 struct Dev {
@@ -448,6 +452,7 @@ __device__ __forceinline__ void sa_body(const Dev &d, const Consts &c, DevState 
         d.nut_new[v] = nn;
         if (p.x <= 0.0 || p.w <= 0.0 || nn < 0.0) {
             atomicMin(&S->err_index, d.orig[v]);
+            S->err_stage = 4;
             S->err_code = FVM_E_NONPOSITIVE;
         }
     }
@@ -468,10 +473,12 @@ __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int
 
 // Monitors and the stop test of one iteration: conres (src/conres.f:11-31),
 // clcd (src/clcd.f:15-46), the RES.DAT row (src/main.f:63) and the loop
-// condition (src/main.f:42, :67-75).  One wide workgroup, so the partials and
-// the solid edges are each read in one pass; fixed reduction order.  Without
-// viscosity the stress terms are zero and are left out.
-constexpr int kFinal = 512;
+// condition (src/main.f:42, :67-75).  One workgroup of 256 threads; fixed
+// reduction order.  Without viscosity the stress terms are zero and are left
+// out.  Inside captured iterations the same body also runs as one extra
+// workgroup of the next iteration's first stage (a work item with patch
+// index -1), so both places use the same width and the same sums.
+constexpr int kFinal = 256;
 
 // force of solid edge b on (xf, yf) from the wall terms its two end points
 // left (src/clcd.f:18-43)
@@ -637,7 +644,16 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     // The solve's status words are loaded with the patch header and tested
     // after the gather, which writes only LDS: their latency hides under it.
     // The header is read through the constant address space (scalar loads).
-    const int stop = S->done | S->err_code;
+    //
+    // A workgroup leaves when the solve is done or an earlier stage kernel
+    // raised an error; an error of this same stage does not stop the others,
+    // so every workgroup reports its lowest failing edge or vertex and the
+    // index is the reference's (the first in its loop order), whatever the
+    // order in which workgroups run.  The second stage first moves a
+    // first-stage error to err_code.
+    const int es = S->err_stage;
+    const int stop = S->done | (es != 0 && es != irk);
+    const int e1 = (MODE == 0 && irk == 2) ? S->err1_code : 0;
     const int hb = kPatchHdr * b;
     auto hdr = [&](int k) { return uniform_at(pa.hdr, hb + k); };
     const int v0 = hdr(PH_V0), NP = hdr(PH_NP), NL = hdr(PH_NL), NG = hdr(PH_NG);
@@ -724,6 +740,14 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     }
     __syncthreads();
     if (stop) return;
+    if (e1) {
+        if (t == 0) {
+            atomicMin(&S->err_index, S->err1_index);
+            S->err_stage = 1;
+            S->err_code = e1;
+        }
+        return;
+    }
 
     // Solid and farfield fluxes need no gradient: they are evaluated during
     // the gradient phase by threads from the top of the workgroup (which the
@@ -889,8 +913,14 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     if (!bnd_early)
         for (int kb = t; kb < NB; kb += kBlock) bnd_flux(kb);
     if (bad != INT_MAX) {
-        atomicMin(&S->err_index, bad);
-        S->err_code = FVM_E_NEG_SOUND;
+        if (MODE == 0 && irk == 1) {
+            atomicMin(&S->err1_index, bad);
+            S->err1_code = FVM_E_NEG_SOUND;
+        } else {
+            atomicMin(&S->err_index, bad);
+            S->err_stage = irk;
+            S->err_code = FVM_E_NEG_SOUND;
+        }
     }
     __syncthreads();
 
@@ -957,6 +987,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
                 if (!TURB) {
                     if (q.x <= 0.0 || q.w <= 0.0) {
                         atomicMin(&S->err_index, d.orig[v]);
+                        S->err_stage = irk;
                         S->err_code = FVM_E_NONPOSITIVE;
                     }
                     // time_step of the next iteration (src/time_step.f:13-28)
@@ -1083,6 +1114,13 @@ k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work
 {
     const int2 w = uniform_at(work, logical_block(nwork, xcd));
     const BatchMember &M = uniform_at(mem, w.x);
+    if (MODE == 0 && !TURB && w.y < 0) {
+        // the monitors of the member's previous iteration: they read only
+        // what that iteration's last stage left, and this stage writes only
+        // its own buffer, so they run beside it
+        final_body<VISC>(M.d, M.c, M.S);
+        return;
+    }
     patch_body<FLUX, LIM, VISC, MODE, TURB, PREC>(M.d, M.c, M.S, M.pa, w.y, irk, M.pbuf[irk - 1],
                                                   M.pbuf[irk % 3], nullptr);
 }
@@ -1119,7 +1157,9 @@ template <bool COMPAT>
 __device__ __forceinline__ void sa_patch_body(const Dev &d, const Consts &c, DevState *S,
                                               const PatchArgs &pa, const int b)
 {
-    const int stop = S->done | S->err_code;
+    // as in the stage kernel, an error of this step does not stop the others
+    const int es = S->err_stage;
+    const int stop = S->done | (es != 0 && es != 4);
     const int hb = kPatchHdr * b;
     auto hdr = [&](int k) { return uniform_at(pa.hdr, hb + k); };
     const int v0 = hdr(PH_V0), NP = hdr(PH_NP), NG = hdr(PH_NG);
@@ -1239,6 +1279,7 @@ __device__ __forceinline__ void sa_patch_body(const Dev &d, const Consts &c, Dev
     d.nut_new[v] = nn;
     if (p.x <= 0.0 || p.w <= 0.0 || nn < 0.0) {
         atomicMin(&S->err_index, d.orig[v]);
+        S->err_stage = 4;
         S->err_code = FVM_E_NONPOSITIVE;
     }
 }
@@ -1307,6 +1348,7 @@ struct fvm_ctx {
     std::vector<int> perm;            // device vertex -> caller's vertex (0-based)
     BatchMember *self_mem = nullptr;  // this solve as a batch of one
     int2 *self_work = nullptr;
+    int2 *self_work_mon = nullptr;    // first-stage list with the monitor item
     std::vector<int> patch_cost;      // per patch: edges + local vertices (work estimate)
     struct fvm_batch *batch = nullptr;   // the batch it belongs to, if any
 };
@@ -1318,6 +1360,7 @@ struct fvm_batch {
     std::vector<fvm_ctx *> mem;
     BatchMember *d_mem = nullptr;
     int2 *d_work = nullptr, *d_vwork = nullptr;
+    int2 *d_work_mon = nullptr;       // first-stage list with one monitor item per member
     DevState *d_states = nullptr;
     int nwork = 0, nvwork = 0, lds = 0, sa_lds = 0, max_nff = 0;
     bool dirty = true, any_vortex = false;
@@ -1511,6 +1554,24 @@ void order_work(std::vector<int2> &work, const std::vector<int> &cost, int cus, 
     work.swap(out);
 }
 
+// The first-stage work list of a captured iteration that follows another:
+// the patches plus one item (member, -1) per member, which runs the monitors
+// of the member's previous iteration beside the stage; costed at a quarter of
+// a mean patch and ordered like the patches.
+std::vector<int2> with_monitors(std::vector<int2> work, std::vector<int> cost, int nm, int cus,
+                                int per_cu)
+{
+    long long sum = 0;
+    for (int c : cost) sum += c;
+    const int mc = std::max(1, (int)(sum / (long long)std::max<size_t>(cost.size(), 1) / 4));
+    for (int i = 0; i < nm; ++i) {
+        work.push_back(make_int2(i, -1));
+        cost.push_back(mc);
+    }
+    order_work(work, cost, cus, per_cu);
+    return work;
+}
+
 int effective_per_cu(const fvm_ctx *x, int lds_bytes)
 {
     return std::max(1, std::min(x->per_cu, x->lds_per_cu / (lds_bytes + 1024)));
@@ -1593,11 +1654,21 @@ void launch_residual(fvm_ctx *x, DevState *S, const double4 *pin, double4 *res)
     });
 }
 
-StageLaunch self_stage(const fvm_ctx *x)
+StageLaunch self_stage(const fvm_ctx *x, bool monitors = false)
 {
     const fvm_params &q = x->par;
-    return StageLaunch{x->self_mem, x->self_work, x->pa.npatch, x->d.xcd, x->pa.lds_bytes, x->stream,
+    return StageLaunch{x->self_mem, monitors ? x->self_work_mon : x->self_work,
+                       x->pa.npatch + (monitors ? 1 : 0), x->d.xcd, x->pa.lds_bytes, x->stream,
                        q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision};
+}
+
+// Captured iterations after the first may run the monitors inside the next
+// first stage: not with SA (its step and copy come between the stages and
+// the monitors) nor with the point-vortex farfield (the next stage reads the
+// monitors' cl).
+bool monitors_merge(const fvm_params &q, bool vortex)
+{
+    return q.iflow != 3 && !vortex;
 }
 
 BatchMember member_of(const fvm_ctx *x)
@@ -1630,7 +1701,10 @@ void launch_tstep(fvm_ctx *x)
 // each launch is bracketed by events and its duration added to the tally.
 struct Tally { std::vector<double> ms; std::vector<int> cnt; std::vector<const char *> names; };
 
-void one_iteration(fvm_ctx *x, Tally *tally)
+// One iteration.  mon_prev: the first stage also runs the monitors of the
+// previous iteration (which then had none of its own); final: the iteration
+// ends with its own monitors.
+void one_iteration(fvm_ctx *x, Tally *tally, bool mon_prev = false, bool final = true)
 {
     const int np = x->np;
     bool visc = x->par.iflow != 1, turb = x->par.iflow == 3;
@@ -1659,7 +1733,7 @@ void one_iteration(fvm_ctx *x, Tally *tally)
             double4 *pout = x->pbuf[irk % 3];
             const int mode = irk == 3 ? 1 : 0;
             (void)pin; (void)pout;
-            timed("k_patch", [&] { launch_stage(self_stage(x), irk, mode); });
+            timed("k_patch", [&] { launch_stage(self_stage(x, mon_prev && irk == 1), irk, mode); });
         }
     } else {
     timed("k_tstep", [&] { launch_tstep(x); });
@@ -1703,7 +1777,7 @@ void one_iteration(fvm_ctx *x, Tally *tally)
             hipLaunchKernelGGL(k_copy, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d.nut_new, x->d.nut, np, x->S);
         });
     }
-    timed("k_final", [&] {
+    if (final) timed("k_final", [&] {
         if (x->par.iflow == 1)
             hipLaunchKernelGGL(k_final<false>, dim3(1), dim3(kFinal), 0, x->stream, x->d, x->c, x->S);
         else
@@ -1746,7 +1820,8 @@ int build_graph(fvm_ctx *x, int iters, hipGraphExec_t *out)
 {
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return FVM_E_HIP;
-    for (int i = 0; i < iters; ++i) one_iteration(x, nullptr);
+    const bool merge = x->variant == 3 && monitors_merge(x->par, x->c.vortex == 1 && x->c.mach_inf < 1.0 && x->nff > 0);
+    for (int i = 0; i < iters; ++i) one_iteration(x, nullptr, merge && i > 0, !merge || i == iters - 1);
     if (hipStreamEndCapture(x->stream, &g) != hipSuccess) return FVM_E_HIP;
     hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -1759,11 +1834,13 @@ int reset_state(fvm_ctx *x)
     s.iter = x->par.iterlast;
     s.fres = x->par.minres + 1.0;
     s.err_index = INT_MAX;
+    s.err1_index = INT_MAX;
     if (!(x->par.maxiter > 0)) { s.done = 1; s.status = FVM_ITER_LIMIT; }
     x->hS = s;
     HIPCHK(hipMemcpyAsync(x->S, &s, sizeof s, hipMemcpyHostToDevice, x->stream));
     DevState h{};
     h.err_index = INT_MAX;
+    h.err1_index = INT_MAX;
     HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
     if (x->nff > 0)       // COMMON cl starts at zero
         hipLaunchKernelGGL(k_farfield_state, dim3(nblk(x->nff)), dim3(kBlock), 0, x->stream, x->d, x->c, 0.0);
@@ -2457,8 +2534,11 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     {
         std::vector<int2> w(x->pa.npatch);
         for (int k = 0; k < x->pa.npatch; ++k) w[k] = make_int2(0, k);
-        order_work(w, x->patch_cost, x->cus, effective_per_cu(x, x->pa.lds_bytes));
+        const int pcu = effective_per_cu(x, x->pa.lds_bytes);
+        const std::vector<int2> wm = with_monitors(w, x->patch_cost, 1, x->cus, pcu);
+        order_work(w, x->patch_cost, x->cus, pcu);
         TRY(upload(x, (int2 **)&x->self_work, w));
+        TRY(upload(x, (int2 **)&x->self_work_mon, wm));
     }
 #undef TRY
 
@@ -2611,6 +2691,7 @@ int fvm_gpu_residu(fvm_ctx *x, const double *prim, const double *nut, double *re
     // the farfield vortex correction
     DevState h{};
     h.err_index = INT_MAX;
+    h.err1_index = INT_MAX;
     h.cl = x->hS.cl;
     HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
     if (x->variant == 3) {
@@ -2725,9 +2806,10 @@ void batch_free(fvm_batch *b)
     if (b->graph_n) (void)hipGraphExecDestroy(b->graph_n);
     if (b->graph_1) (void)hipGraphExecDestroy(b->graph_1);
     b->graph_n = b->graph_1 = nullptr;
-    for (void *p : {(void *)b->d_mem, (void *)b->d_work, (void *)b->d_vwork, (void *)b->d_states})
+    for (void *p : {(void *)b->d_mem, (void *)b->d_work, (void *)b->d_vwork, (void *)b->d_work_mon,
+                    (void *)b->d_states})
         if (p) (void)hipFree(p);
-    b->d_mem = nullptr; b->d_work = b->d_vwork = nullptr; b->d_states = nullptr;
+    b->d_mem = nullptr; b->d_work = b->d_vwork = b->d_work_mon = nullptr; b->d_states = nullptr;
     b->nwork = b->nvwork = 0;
 }
 
@@ -2764,9 +2846,15 @@ int batch_rebuild(fvm_batch *b)
         b->max_nff = std::max(b->max_nff, x->nff);
         if (x->c.vortex == 1 && x->c.mach_inf < 1.0) b->any_vortex = true;
     }
-    if (!b->mem.empty()) order_work(work, cost, b->mem[0]->cus, effective_per_cu(b->mem[0], b->lds));
+    std::vector<int2> work_mon;
+    if (!b->mem.empty()) {
+        const int cus = b->mem[0]->cus, pcu = effective_per_cu(b->mem[0], b->lds);
+        work_mon = with_monitors(work, cost, (int)b->mem.size(), cus, pcu);
+        order_work(work, cost, cus, pcu);
+    }
     int rc = batch_upload(&b->d_mem, m);
     if (!rc) rc = batch_upload(&b->d_work, work);
+    if (!rc) rc = batch_upload(&b->d_work_mon, work_mon);
     if (!rc) rc = batch_upload(&b->d_vwork, vwork);
     if (!rc && hipMalloc((void **)&b->d_states, std::max<size_t>(m.size(), 1) * sizeof(DevState)) != hipSuccess)
         rc = FVM_E_NOMEM;
@@ -2779,7 +2867,8 @@ int batch_rebuild(fvm_batch *b)
     return FVM_OK;
 }
 
-void batch_iteration(fvm_batch *b)
+// one iteration of every member; mon_prev and final as in one_iteration
+void batch_iteration(fvm_batch *b, bool mon_prev = false, bool final = true)
 {
     const fvm_params &q = b->mem[0]->par;
     const int nm = (int)b->mem.size();
@@ -2788,7 +2877,10 @@ void batch_iteration(fvm_batch *b)
         hipLaunchKernelGGL(k_tstep_batch<true>, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
     StageLaunch L{b->d_mem, b->d_work, b->nwork, 1, b->lds, b->stream,
                   q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision};
-    for (int irk = 1; irk <= 3; ++irk) launch_stage(L, irk, irk == 3 ? 1 : 0);
+    StageLaunch LM = L;
+    LM.work = b->d_work_mon;
+    LM.nwork = b->nwork + nm;
+    for (int irk = 1; irk <= 3; ++irk) launch_stage(mon_prev && irk == 1 ? LM : L, irk, irk == 3 ? 1 : 0);
     if (turb) {
         if (q.sa_compat)
             hipLaunchKernelGGL(k_sa_patch_batch<true>, dim3(b->nwork), dim3(kBlock), b->sa_lds, b->stream,
@@ -2798,6 +2890,7 @@ void batch_iteration(fvm_batch *b)
                                b->d_mem, b->d_work, b->nwork, 1);
         hipLaunchKernelGGL(k_copy_batch, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
     }
+    if (!final) return;
     if (q.iflow == 1)
         hipLaunchKernelGGL(k_final_batch<false>, dim3(nm), dim3(kFinal), 0, b->stream, b->d_mem);
     else
@@ -2810,7 +2903,8 @@ int batch_graph(fvm_batch *b, int iters, hipGraphExec_t *out)
 {
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return FVM_E_HIP;
-    for (int i = 0; i < iters; ++i) batch_iteration(b);
+    const bool merge = monitors_merge(b->mem[0]->par, b->any_vortex && b->max_nff > 0);
+    for (int i = 0; i < iters; ++i) batch_iteration(b, merge && i > 0, !merge || i == iters - 1);
     if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return FVM_E_HIP;
     hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);

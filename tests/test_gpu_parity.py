@@ -136,6 +136,30 @@ def test_negative_sound_speed_stops_like_reference():
     assert ei.value.index == o.error_index()
 
 
+def test_error_index_in_multi_round_launch():
+    """Negative pressures at vertices far apart on the 1025x257 grid (about
+    four thousand patches: several rounds of workgroups per stage): every
+    workgroup reports its failing edges, so the index is the reference's
+    first failing edge whatever the order in which workgroups run."""
+    from oracle.oracle import Oracle
+    g = mesh.naca0012_1025x257()
+    kw = dict(iflow=1, iflux=1, ilimit=1, mach_inf=0.5, aoa_deg=0.0, cfl=1.0, maxiter=100)
+    o = Oracle(g.coord, g.elem, g.ptype, **kw)
+    prim = o.get("prim").copy()
+    for v in (250000, 170000, 90000, 1000):
+        prim[4 * v + 3] = -50.0
+    o.set("prim", prim)
+    _, st = o.iterate(1)
+    assert st == -4
+    with Solver(g, Inputs.from_kwargs(**kw)) as s:
+        assert s.layout_info()["patches"] > 4 * 256
+        s.set_state(prim)
+        with pytest.raises(FvmError) as ei:
+            s.iterate(3)
+    assert ei.value.code == -4
+    assert ei.value.index == o.error_index()
+
+
 def test_convergence_stop_and_no_work_after():
     name = "euler_roe_transonic_65x17"
     fx = fixture(name)
