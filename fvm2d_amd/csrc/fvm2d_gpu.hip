@@ -863,7 +863,10 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             s_gx[g] = GT{qx.x, qx.y, qx.z, qx.w};
             s_gy[g] = GT{qy.x, qy.y, qy.z, qy.w};
         }
-        if (MODE != 0 && g < NP) { d.qx[v0 + g] = qx; d.qy[v0 + g] = qy; }
+        // the gradients a caller may read: the residual hook's, and the last
+        // stage's where the SA step or the wall stresses read them; without
+        // viscosity the host recomputes them on request (same operations)
+        if ((MODE == 2 || (MODE == 1 && VISC)) && g < NP) { d.qx[v0 + g] = qx; d.qy[v0 + g] = qy; }
     }
     const int tl = t;                             // local vertex this thread updates
     __syncthreads();
@@ -1349,6 +1352,9 @@ struct fvm_ctx {
     BatchMember *self_mem = nullptr;  // this solve as a batch of one
     int2 *self_work = nullptr;
     int2 *self_work_mon = nullptr;    // first-stage list with the monitor item
+    // inviscid stage kernels leave qx, qy unwritten; true once iterations ran
+    // since they were last computed (fvm_gpu_get_state then recomputes them)
+    bool qxy_stale = false;
     std::vector<int> patch_cost;      // per patch: edges + local vertices (work estimate)
     struct fvm_batch *batch = nullptr;   // the batch it belongs to, if any
 };
@@ -2660,6 +2666,17 @@ int fvm_gpu_set_cl(fvm_ctx *x, double cl)
 int fvm_gpu_get_state(fvm_ctx *x, double *prim, double *nut, double *qx, double *qy, double *mu)
 {
     const size_t np = (size_t)x->np;
+    if (x->qxy_stale && (qx || qy)) {
+        // the gradients of the last residual: those of the last stage's
+        // input (buffer C), through the residual kernel, which writes them
+        DevState h{};
+        h.err_index = INT_MAX;
+        h.err1_index = INT_MAX;
+        HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
+        launch_residual(x, x->S_hook, x->pbuf[2], x->res_tmp);
+        HIPCHK(hipGetLastError());
+        x->qxy_stale = false;
+    }
     std::vector<double> a(4 * np), b(4 * np), q(4 * np), r(np), m(np);
     HIPCHK(hipMemcpyAsync(a.data(), x->d.prim, 32 * np, hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipMemcpyAsync(r.data(), x->d.nut, 8 * np, hipMemcpyDeviceToHost, x->stream));
@@ -2696,8 +2713,10 @@ int fvm_gpu_residu(fvm_ctx *x, const double *prim, const double *nut, double *re
     HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
     if (x->variant == 3) {
         launch_residual(x, x->S_hook, x->d.prim, x->res_tmp);
+        x->qxy_stale = false;                   // the hook's gradients, as in the reference
     } else {
         residual_front(x, x->S_hook);
+        x->qxy_stale = false;
         if (x->par.iflow != 1)
             hipLaunchKernelGGL(k_res<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->res_tmp);
         else
@@ -2723,6 +2742,7 @@ int fvm_gpu_iterate(fvm_ctx *x, int n, double *hist, int *ndone, int *err_index)
     while (done_total < n && !x->hS.done) {
         int chunk = std::min(n - done_total, kHistCap);
         hipLaunchKernelGGL(k_begin_chunk, dim3(1), dim3(1), 0, x->stream, x->S);
+        x->qxy_stale = x->variant == 3 && x->par.iflow == 1;
         if (x->graphs_ok && !x->graph_n) {
             if (build_graph(x, kGraphIters, &x->graph_n) || build_graph(x, 1, &x->graph_1)) {
                 x->graphs_ok = false;
@@ -2759,6 +2779,7 @@ int fvm_gpu_time_kernels(fvm_ctx *x, int n, double *ms, const char **names, int 
     Tally t;
     for (int i = 0; i < n && !x->hS.done; ++i) {
         hipLaunchKernelGGL(k_begin_chunk, dim3(1), dim3(1), 0, x->stream, x->S);
+        x->qxy_stale = x->variant == 3 && x->par.iflow == 1;
         one_iteration(x, &t);
         HIPCHK(hipMemcpyAsync(&x->hS, x->S, sizeof(DevState), hipMemcpyDeviceToHost, x->stream));
         HIPCHK(hipStreamSynchronize(x->stream));
@@ -2986,6 +3007,7 @@ int fvm_batch_launch(fvm_batch *b, int n)
     bool running = false;
     for (const fvm_ctx *x : b->mem) running = running || !x->hS.done;
     if (!running) return FVM_OK;
+    for (fvm_ctx *x : b->mem) x->qxy_stale = x->par.iflow == 1;      // stage kernels only
     hipLaunchKernelGGL(k_begin_batch, dim3(nblk(nm)), dim3(kBlock), 0, b->stream, b->d_mem, nm);
     if (b->graphs_ok && !b->graph_n) {
         if (batch_graph(b, kGraphIters, &b->graph_n) || batch_graph(b, 1, &b->graph_1)) {
