@@ -81,7 +81,7 @@ def _cpu_sample(args) -> float:
 
 
 def cpu_baseline(grid, kw, target_s: float = 12.0, procs: int = 16) -> dict:
-    """The oracle (a serial C restatement of the CPU solver, gcc -O2) on the
+    """The oracle (a serial C restatement of the CPU solver, gcc -O3) on the
     host: iterations of the same workload from the free stream, on one core,
     then as independent processes pinned one per host CPU (up to procs, within
     the CPUs this process may use), run at the same time.  Runs before the
@@ -101,7 +101,7 @@ def cpu_baseline(grid, kw, target_s: float = 12.0, procs: int = 16) -> dict:
     out = {"value": grid.np * h.shape[0] / dt, "unit": "vertex-iterations/s", "cores": 1,
            "kind": "port",
            "sample": f"{h.shape[0]} iterations of the same 513x129 Euler/Roe workload from the "
-                     f"free stream, serial C oracle (gcc -O2 -ffp-contract=off) on {model}, "
+                     f"free stream, serial C oracle (gcc -O3 -ffp-contract=off) on {model}, "
                      f"{dt:.1f} s"}
     cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else []
     k = min(procs, len(cpus))
