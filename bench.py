@@ -56,6 +56,16 @@ def traffic_from_profiles(kernel: str):
         return None
 
 
+def stream_copy_gbs(nbytes: int = 1 << 30, reps: int = 20):
+    """Achievable HBM bandwidth: a plain 16-byte copy kernel between two
+    buffers of nbytes (fvm_gpu_stream_copy), read + write bytes per second."""
+    import ctypes
+    from fvm2d_amd import gpu
+    v = ctypes.c_double(0.0)
+    rc = gpu.lib().fvm_gpu_stream_copy(nbytes, reps, ctypes.byref(v))
+    return round(v.value, 1) if rc == 0 else None
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -403,10 +413,16 @@ def main() -> int:
         B = algorithmic_bytes(dom, npt, nt, ne)
         if B is not None:
             achieved = B / (t_ms * 1e-3) / 1e9
+            traffic = traffic_from_profiles(dom)
+            copy = stream_copy_gbs()
             roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": traffic_from_profiles(dom),
+                        "traffic": traffic,
+                        "traffic_rate_gbs": (round(traffic / (t_ms * 1e-3) / 1e9, 1)
+                                             if traffic else None),
+                        "achievable_copy_gbs": copy,
+                        "frac_of_copy": round(achieved / copy, 4) if copy else None,
                         "kernel_ms_per_launch": round(t_ms, 5),
                         "kernel_ms": {k: round(v, 5) for k, v in ms.items()},
                         "algorithmic_bytes_per_launch": B}

@@ -1068,6 +1068,16 @@ __global__ void __launch_bounds__(kBlock) k_wall_dist(int np, const double2 *__r
 
 __global__ void k_begin_chunk(DevState *S) { S->nhist = 0; }
 
+// plain copy for the achievable-bandwidth figure: 16 bytes per thread and
+// step, grid-stride
+__global__ void __launch_bounds__(kBlock) k_stream_copy(const double2 *__restrict__ src,
+                                                        double2 *__restrict__ dst, long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
 // far-field states of all farfield edges for a given cl
 __global__ void k_farfield_state(Dev d, Consts c, double cl)
 {
@@ -2790,6 +2800,39 @@ int fvm_gpu_time_kernels(fvm_ctx *x, int n, double *ms, const char **names, int 
         if (names) names[i] = t.names[i];
     }
     return k;
+}
+
+int fvm_gpu_stream_copy(long long bytes, int reps, double *gbs)
+{
+    *gbs = 0.0;
+    if (bytes < 16 || reps < 1) return FVM_E_INPUT;
+    const long long n = bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    if (hipMalloc((void **)&a, n * 16) != hipSuccess) return FVM_E_NOMEM;
+    if (hipMalloc((void **)&b, n * 16) != hipSuccess) { (void)hipFree(a); return FVM_E_NOMEM; }
+    int rc = FVM_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int grid = 256 * 8 * 4;                 // 8 workgroups of 256 per CU, 4 rounds
+    float t = 0.f;
+    if (hipMemset(a, 0, n * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = FVM_E_HIP;
+    } else {
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kBlock), 0, nullptr, a, b, n);   // warm
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kBlock), 0, nullptr, a, b, n);
+        (void)hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess ||
+            hipGetLastError() != hipSuccess)
+            rc = FVM_E_HIP;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (rc == FVM_OK && t > 0.f) *gbs = 2.0 * (double)(n * 16) * reps / ((double)t * 1e-3) / 1e9;
+    return rc;
 }
 
 int fvm_gpu_time_stage(fvm_ctx *x, int n, double *ms)

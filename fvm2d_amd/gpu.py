@@ -86,6 +86,8 @@ def lib():
     L.fvm_gpu_time_kernels.argtypes = [vp, ci, vp, ctypes.POINTER(ctypes.c_char_p), ci]
     L.fvm_gpu_time_stage.restype = ci
     L.fvm_gpu_time_stage.argtypes = [vp, ci, ctypes.POINTER(ctypes.c_double)]
+    L.fvm_gpu_stream_copy.restype = ci
+    L.fvm_gpu_stream_copy.argtypes = [ctypes.c_longlong, ci, ctypes.POINTER(ctypes.c_double)]
     L.fvm_patch_plan.restype = ci
     L.fvm_patch_plan.argtypes = [ctypes.POINTER(Mesh), ci, ci, ci, ci, vp]
     L.fvm_gpu_layout_info.argtypes = [vp, vp]

@@ -32,3 +32,15 @@ def test_autotune_chooses_and_caches(tmp_path):
             states.append((h, s.get_state()["prim"]))
     np.testing.assert_array_equal(states[0][1], states[1][1])
     np.testing.assert_array_equal(states[0][0][:, [0, 2, 3, 4, 5]], states[1][0][:, [0, 2, 3, 4, 5]])
+
+
+def test_stream_copy_bandwidth_is_plausible():
+    """The achievable-bandwidth figure the stage kernel is read against: a
+    plain copy of 256 MiB reaches a large share of the 8 TB/s peak, and never
+    more than the peak."""
+    import ctypes
+    from fvm2d_amd import gpu
+    v = ctypes.c_double(0.0)
+    assert gpu.lib().fvm_gpu_stream_copy(256 << 20, 5, ctypes.byref(v)) == 0
+    assert 1000.0 < v.value < 8000.0
+    assert gpu.lib().fvm_gpu_stream_copy(8, 5, ctypes.byref(v)) != 0     # below one 16-byte item
