@@ -151,11 +151,12 @@ int fvm_gpu_time_kernels(fvm_ctx *ctx, int n, double *ms, const char **names, in
 int fvm_gpu_time_stage(fvm_ctx *ctx, int n, double *ms);
 
 /* Achievable HBM bandwidth on the current device: a plain copy kernel
- * (16-byte loads and stores, grid-stride) between two fresh buffers of
- * `bytes` each, timed over `reps` launches between HIP events; *gbs =
- * 2 * bytes per launch / mean duration, in GB/s.  Allocates and frees the two
- * buffers.  No counterpart in the reference; the figure the stage kernel's
- * bandwidth is read against beside the 8 TB/s peak. */
+ * (16-byte loads and stores, four workgroups per CU striding over the
+ * buffers) between two fresh buffers of `bytes` each, timed over `reps`
+ * launches between HIP events; *gbs = 2 * bytes per launch / mean duration,
+ * in GB/s.  Allocates and frees the two buffers.  No counterpart in the
+ * reference; the figure the stage kernel's bandwidth is read against beside
+ * the 8 TB/s peak. */
 int fvm_gpu_stream_copy(long long bytes, int reps, double *gbs);
 
 /* Tuning knobs (no counterpart in the reference; used by the autotuner):
