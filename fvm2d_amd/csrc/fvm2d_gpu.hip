@@ -43,8 +43,16 @@ namespace {
 
 constexpr int kBlock = 256;
 // waves per SIMD the stage kernel is compiled for (one workgroup of
-// kBlock threads puts one wave on each SIMD of a CU)
-constexpr int kPatchWavesEuler = 4, kPatchWavesVisc = 3, kPatchWavesSplit = 4;
+// kBlock threads puts one wave on each SIMD of a CU): 4 (128 VGPRs) but for
+// SA, whose instance keeps 3 (168 VGPRs; at 4 it spills and its patches shrink
+// to fit four in the LDS: 0.81e9 -> 0.68e9 vertex-iterations/s on the
+// 2049x513 batch, where laminar 1025x257 gains 1.00e9 -> 1.10e9)
+constexpr int kPatchWavesEuler = 4, kPatchWavesLaminar = 4, kPatchWavesSA = 3, kPatchWavesSplit = 4;
+template <bool VISC, bool TURB, int PREC>
+constexpr int patch_waves()
+{
+    return TURB ? kPatchWavesSA : (VISC ? kPatchWavesLaminar : (PREC ? kPatchWavesSplit : kPatchWavesEuler));
+}
 constexpr int kHistCap = 4096;      // history rows per fvm_gpu_iterate chunk
 constexpr int kGraphIters = 8;      // iterations captured per graph
 
@@ -1018,7 +1026,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
 }
 
 template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
-__global__ void __launch_bounds__(kBlock, VISC ? kPatchWavesVisc : (PREC ? kPatchWavesSplit : kPatchWavesEuler))
+__global__ void __launch_bounds__(kBlock, (patch_waves<VISC, TURB, PREC>()))
 k_patch(Dev d, Consts c, DevState *S, PatchArgs pa, int irk, const double4 *__restrict__ pin,
         double4 *__restrict__ pout, double4 *res_out)
 {
@@ -1121,7 +1129,7 @@ struct BatchMember {
 };
 
 template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
-__global__ void __launch_bounds__(kBlock, VISC ? kPatchWavesVisc : (PREC ? kPatchWavesSplit : kPatchWavesEuler))
+__global__ void __launch_bounds__(kBlock, (patch_waves<VISC, TURB, PREC>()))
 k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work, int nwork, int xcd,
               int irk)
 {
@@ -2413,7 +2421,9 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
         (void)hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
     }
     x->cus = cus;
-    x->per_cu = par->iflow != 1 ? kPatchWavesVisc : (par->precision == 1 ? kPatchWavesSplit : kPatchWavesEuler);
+    x->per_cu = par->iflow == 3 ? kPatchWavesSA
+                                : (par->iflow == 2 ? kPatchWavesLaminar
+                                                   : (par->precision == 1 ? kPatchWavesSplit : kPatchWavesEuler));
     Layout L;
     if (par->patch_size > 0) {
         rc = build_layout(m, par->iflow, par->precision, par->patch_size, lds_dev - 6 * 1024, L);
