@@ -1128,8 +1128,11 @@ struct BatchMember {
     double4 *pbuf[3];      // stage buffers O, B, C
 };
 
-template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
-__global__ void __launch_bounds__(kBlock, (patch_waves<VISC, TURB, PREC>()))
+// WIDE: an SA instance compiled for 4 waves/SIMD (128 VGPRs, a few spilled),
+// launched when four workgroups fit a CU's LDS (small patches of small
+// meshes, as served solves have); the same operations, so the same bits
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC, bool WIDE = false>
+__global__ void __launch_bounds__(kBlock, (WIDE ? 4 : patch_waves<VISC, TURB, PREC>()))
 k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work, int nwork, int xcd,
               int irk)
 {
@@ -1601,6 +1604,20 @@ int effective_per_cu(const fvm_ctx *x, int lds_bytes)
     return std::max(1, std::min(x->per_cu, x->lds_per_cu / (lds_bytes + 1024)));
 }
 
+// SA solves (Roe, limited) whose patches leave room for four workgroups a CU
+// run the 4-wave instance of the stage kernel
+bool wide_sa(const fvm_ctx *x, int lds_bytes)
+{
+    const fvm_params &q = x->par;
+    return q.iflow == 3 && q.iflux == 1 && q.ilimit == 1 && (lds_bytes + 1024) * 4 <= x->lds_per_cu;
+}
+
+// workgroups a CU holds at once for the stage kernel of this solve
+int stage_per_cu(const fvm_ctx *x, int lds_bytes)
+{
+    return wide_sa(x, lds_bytes) ? 4 : effective_per_cu(x, lds_bytes);
+}
+
 // Stage kernels read the stage state `pin` and write `pout`; the buffers
 // rotate O -> B -> C -> O over the three stages, O being the state at the
 // start of the iteration (which the RK combination also reads).
@@ -1648,6 +1665,7 @@ struct StageLaunch {
     int nwork, xcd, lds;
     hipStream_t stream;
     int iflow, iflux, hllc_rotate, ilimit, precision;
+    bool wide;                  // SA, Roe, limited: the 4-wave instance
 };
 
 void launch_stage(const StageLaunch &b, int irk, int mode)
@@ -1656,6 +1674,17 @@ void launch_stage(const StageLaunch &b, int irk, int mode)
               [&](auto fl, auto lm, auto vi, auto tu, auto pr) {
         constexpr int F = decltype(fl)::value, L = decltype(lm)::value, P = decltype(pr)::value;
         constexpr bool V = decltype(vi)::value != 0, T = decltype(tu)::value != 0;
+        if constexpr (T && F == 1 && L == 1) {
+            if (b.wide) {
+                if (mode == 0)
+                    hipLaunchKernelGGL((k_patch_batch<F, L, V, 0, T, P, true>), dim3(b.nwork), dim3(kBlock),
+                                       b.lds, b.stream, b.mem, b.work, b.nwork, b.xcd, irk);
+                else
+                    hipLaunchKernelGGL((k_patch_batch<F, L, V, 1, T, P, true>), dim3(b.nwork), dim3(kBlock),
+                                       b.lds, b.stream, b.mem, b.work, b.nwork, b.xcd, irk);
+                return;
+            }
+        }
         if (mode == 0)
             hipLaunchKernelGGL((k_patch_batch<F, L, V, 0, T, P>), dim3(b.nwork), dim3(kBlock), b.lds,
                                b.stream, b.mem, b.work, b.nwork, b.xcd, irk);
@@ -1683,7 +1712,8 @@ StageLaunch self_stage(const fvm_ctx *x, bool monitors = false)
     const fvm_params &q = x->par;
     return StageLaunch{x->self_mem, monitors ? x->self_work_mon : x->self_work,
                        x->pa.npatch + (monitors ? 1 : 0), x->d.xcd, x->pa.lds_bytes, x->stream,
-                       q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision};
+                       q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision,
+                       wide_sa(x, x->pa.lds_bytes)};
 }
 
 // Captured iterations after the first may run the monitors inside the next
@@ -2380,11 +2410,10 @@ int fvm_patch_plan(const fvm_mesh *m, int iflow, int precision, int patch_size, 
 
 int fvm_gpu_layout_info(const fvm_ctx *x, long long *info)
 {
-    const int lds = x->pa.lds_bytes + 1024;
     info[0] = x->pa.npatch;
     info[1] = x->max_patch;
     info[2] = x->pa.lds_bytes;
-    info[3] = std::min(x->per_cu, x->lds_per_cu / lds);
+    info[3] = stage_per_cu(x, x->pa.lds_bytes);
     info[4] = x->cus;
     info[5] = x->target;
     info[6] = x->halo_total;
@@ -2560,7 +2589,7 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     {
         std::vector<int2> w(x->pa.npatch);
         for (int k = 0; k < x->pa.npatch; ++k) w[k] = make_int2(0, k);
-        const int pcu = effective_per_cu(x, x->pa.lds_bytes);
+        const int pcu = stage_per_cu(x, x->pa.lds_bytes);
         const std::vector<int2> wm = with_monitors(w, x->patch_cost, 1, x->cus, pcu);
         order_work(w, x->patch_cost, x->cus, pcu);
         TRY(upload(x, (int2 **)&x->self_work, w));
@@ -2929,7 +2958,7 @@ int batch_rebuild(fvm_batch *b)
     }
     std::vector<int2> work_mon;
     if (!b->mem.empty()) {
-        const int cus = b->mem[0]->cus, pcu = effective_per_cu(b->mem[0], b->lds);
+        const int cus = b->mem[0]->cus, pcu = stage_per_cu(b->mem[0], b->lds);
         work_mon = with_monitors(work, cost, (int)b->mem.size(), cus, pcu);
         order_work(work, cost, cus, pcu);
     }
@@ -2957,7 +2986,7 @@ void batch_iteration(fvm_batch *b, bool mon_prev = false, bool final = true)
     if (turb)
         hipLaunchKernelGGL(k_tstep_batch<true>, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
     StageLaunch L{b->d_mem, b->d_work, b->nwork, 1, b->lds, b->stream,
-                  q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision};
+                  q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision, wide_sa(b->mem[0], b->lds)};
     StageLaunch LM = L;
     LM.work = b->d_work_mon;
     LM.nwork = b->nwork + nm;
