@@ -1138,7 +1138,7 @@ k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work
 {
     const int2 w = uniform_at(work, logical_block(nwork, xcd));
     const BatchMember &M = uniform_at(mem, w.x);
-    if (MODE == 0 && !TURB && w.y < 0) {
+    if (MODE == 0 && w.y < 0) {
         // the monitors of the member's previous iteration: they read only
         // what that iteration's last stage left, and this stage writes only
         // its own buffer, so they run beside it
@@ -1717,12 +1717,13 @@ StageLaunch self_stage(const fvm_ctx *x, bool monitors = false)
 }
 
 // Captured iterations after the first may run the monitors inside the next
-// first stage: not with SA (its step and copy come between the stages and
-// the monitors) nor with the point-vortex farfield (the next stage reads the
-// monitors' cl).
+// first stage, except with the point-vortex farfield (the next stage reads
+// the monitors' cl).  With SA the next iteration's time step comes first:
+// it writes only dt, which a stopped solve no longer uses.
 bool monitors_merge(const fvm_params &q, bool vortex)
 {
-    return q.iflow != 3 && !vortex;
+    (void)q;
+    return !vortex;
 }
 
 BatchMember member_of(const fvm_ctx *x)
