@@ -95,14 +95,19 @@ struct Dev {
 
 // ----------------------------------------------------------------- kernels
 
-// save_old + time_step, src/save_old.f:11-15, src/time_step.f:13-28
-template <bool VISC>
+// save_old + time_step, src/save_old.f:11-15, src/time_step.f:13-28.
+// SAVE: the edge-flux layout keeps a separate prim_old; the stage kernels
+// read the iteration-start buffer itself and need no copy.  COPY: first the
+// previous iteration's nu_tilde update (nut = nut_new), which captured SA
+// iterations fold in here instead of a kernel of its own.
+template <bool VISC, bool SAVE = true, bool COPY = false>
 __device__ __forceinline__ void tstep_body(const Dev &d, const Consts &c, int v)
 {
     if (d.S->done) return;
     if (v >= d.np) return;
+    if (COPY) d.nut[v] = d.nut_new[v];
     double4 p = d.prim[v];
-    d.prim_old[v] = p;
+    if (SAVE) d.prim_old[v] = p;
     double q = sqrt(sq(p.y) + sq(p.z));
     double a = sqrt(c.GAMMA * p.w / p.x);
     double dr = d.drmin[v];
@@ -114,10 +119,10 @@ __device__ __forceinline__ void tstep_body(const Dev &d, const Consts &c, int v)
     }
 }
 
-template <bool VISC>
+template <bool VISC, bool SAVE = true, bool COPY = false>
 __global__ void __launch_bounds__(kBlock) k_tstep(Dev d, Consts c)
 {
-    tstep_body<VISC>(d, c, blockIdx.x * blockDim.x + threadIdx.x);
+    tstep_body<VISC, SAVE, COPY>(d, c, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Sutherland viscosity and total viscosity, src/residu.f:23-26
@@ -1150,13 +1155,13 @@ k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work
 }
 
 // vertex kernels over a work list of (member, block of kBlock vertices)
-template <bool VISC>
+template <bool VISC, bool COPY>
 __global__ void __launch_bounds__(kBlock) k_tstep_batch(const BatchMember *__restrict__ mem,
                                                         const int2 *__restrict__ vwork)
 {
     const int2 w = uniform_at(vwork, (int)blockIdx.x);
     const BatchMember &M = uniform_at(mem, w.x);
-    tstep_body<VISC>(M.d, M.c, w.y * kBlock + threadIdx.x);
+    tstep_body<VISC, false, COPY>(M.d, M.c, w.y * kBlock + threadIdx.x);
 }
 
 __global__ void __launch_bounds__(kBlock) k_copy_batch(const BatchMember *__restrict__ mem,
@@ -1619,15 +1624,8 @@ int stage_per_cu(const fvm_ctx *x, int lds_bytes)
 }
 
 // Stage kernels read the stage state `pin` and write `pout`; the buffers
-// rotate O -> B -> C -> O over the three stages, O being the state at the
-// start of the iteration (which the RK combination also reads).
-Dev stage_dev(const fvm_ctx *x, const double4 *pin)
-{
-    Dev d = x->d;
-    d.prim_old = x->pbuf[0];
-    d.prim = const_cast<double4 *>(pin);
-    return d;
-}
+// rotate O -> B -> C -> O over the three stages, O (d.prim) being the state
+// at the start of the iteration, which the RK combination also reads.
 
 template <int V> using IC = std::integral_constant<int, V>;
 
@@ -1745,11 +1743,22 @@ int sync_self(fvm_ctx *x)
     return FVM_OK;
 }
 
-void launch_tstep(fvm_ctx *x)
+// time step (and save_old for the edge-flux layout); copy: also the
+// previous iteration's nu_tilde update (SA, stage kernels only)
+void launch_tstep(fvm_ctx *x, bool copy = false)
 {
     const int np = x->np;
-    if (x->par.iflow != 1) hipLaunchKernelGGL(k_tstep<true>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c);
-    else hipLaunchKernelGGL(k_tstep<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c);
+    const dim3 g(nblk(np)), blk(kBlock);
+    if (x->variant != 3) {
+        if (x->par.iflow != 1) hipLaunchKernelGGL((k_tstep<true>), g, blk, 0, x->stream, x->d, x->c);
+        else hipLaunchKernelGGL((k_tstep<false>), g, blk, 0, x->stream, x->d, x->c);
+    } else if (copy) {
+        hipLaunchKernelGGL((k_tstep<true, false, true>), g, blk, 0, x->stream, x->d, x->c);
+    } else if (x->par.iflow != 1) {
+        hipLaunchKernelGGL((k_tstep<true, false>), g, blk, 0, x->stream, x->d, x->c);
+    } else {
+        hipLaunchKernelGGL((k_tstep<false, false>), g, blk, 0, x->stream, x->d, x->c);
+    }
 }
 
 // Kernel sequence of one iteration (src/main.f:44-63).  If `timer` is set,
@@ -1782,7 +1791,9 @@ void one_iteration(fvm_ctx *x, Tally *tally, bool mon_prev = false, bool final =
     if (x->variant == 3) {
         // time_step runs at the end of the previous iteration's last stage,
         // except when the SA step (which reads this iteration's dt) follows
-        if (turb) timed("k_tstep", [&] { launch_tstep(x); });
+        // with SA, iterations after the first of a captured graph also make
+        // the previous iteration's nu_tilde update here (mon_prev marks them)
+        if (turb) timed("k_tstep", [&] { launch_tstep(x, mon_prev); });
         for (int irk = 1; irk <= 3; ++irk) {
             const double4 *pin = x->pbuf[irk - 1];
             double4 *pout = x->pbuf[irk % 3];
@@ -1829,7 +1840,9 @@ void one_iteration(fvm_ctx *x, Tally *tally, bool mon_prev = false, bool final =
             } else {
                 hipLaunchKernelGGL(k_sa<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->c, x->S);
             }
-            hipLaunchKernelGGL(k_copy, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d.nut_new, x->d.nut, np, x->S);
+            if (final || x->variant != 3)
+                hipLaunchKernelGGL(k_copy, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d.nut_new, x->d.nut,
+                                   np, x->S);
         });
     }
     if (final) timed("k_final", [&] {
@@ -2984,8 +2997,12 @@ void batch_iteration(fvm_batch *b, bool mon_prev = false, bool final = true)
     const fvm_params &q = b->mem[0]->par;
     const int nm = (int)b->mem.size();
     const bool turb = q.iflow == 3;
-    if (turb)
-        hipLaunchKernelGGL(k_tstep_batch<true>, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
+    if (turb && mon_prev)      // the previous iteration's nu_tilde update folded in
+        hipLaunchKernelGGL((k_tstep_batch<true, true>), dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem,
+                           b->d_vwork);
+    else if (turb)
+        hipLaunchKernelGGL((k_tstep_batch<true, false>), dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem,
+                           b->d_vwork);
     StageLaunch L{b->d_mem, b->d_work, b->nwork, 1, b->lds, b->stream,
                   q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision, wide_sa(b->mem[0], b->lds)};
     StageLaunch LM = L;
@@ -2999,7 +3016,8 @@ void batch_iteration(fvm_batch *b, bool mon_prev = false, bool final = true)
         else
             hipLaunchKernelGGL(k_sa_patch_batch<false>, dim3(b->nwork), dim3(kBlock), b->sa_lds, b->stream,
                                b->d_mem, b->d_work, b->nwork, 1);
-        hipLaunchKernelGGL(k_copy_batch, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
+        if (final)             // otherwise the next iteration's time step copies
+            hipLaunchKernelGGL(k_copy_batch, dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem, b->d_vwork);
     }
     if (!final) return;
     if (q.iflow == 1)
