@@ -703,7 +703,13 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     // gradients in LDS: FP64, or rounded to FP32 in the split mode, whose
     // reconstruction reads them in FP32 (the FP64 values go to qx, qy)
     using GT = typename std::conditional<PREC == 1, float4, double4>::type;
-    GT *s_gx = reinterpret_cast<GT *>(s_mu + NLa * (TURB ? 4 : (VISC ? 2 : 0)));
+    // inviscid split precision: an FP32 copy of the state and FP32
+    // coordinates relative to the patch's first vertex, for the FP32
+    // gradients and reconstruction (in the bytes the viscous arrays take)
+    constexpr bool F32G = PREC == 1 && !VISC;
+    float4 *s_p32 = reinterpret_cast<float4 *>(s_mu);
+    float2 *s_c32 = reinterpret_cast<float2 *>(s_p32 + NLa);
+    GT *s_gx = reinterpret_cast<GT *>(s_mu + NLa * (TURB ? 4 : (VISC ? 2 : (F32G ? 3 : 0))));
     GT *s_gy = s_gx + NG;
     // (with viscosity the gradients' bytes, at least 40 per triangle touching
     // P, first stage the stress terms); then the gradients' area factor ais
@@ -726,11 +732,17 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     short *s_psl = s_gtl + NGTL;
 
     // 1. stage state of P, H1, H2
+    const double2 xref = F32G ? uniform_at(d.coord, v0) : make_double2(0.0, 0.0);
     for (int i = t; i < NL; i += kBlock) {
         const int gv = i < NP ? v0 + i : halo[i - NP];
         double4 p = pin[gv];
+        const double2 xy = d.coord[gv];
         s_prim[i] = p;
-        s_coord[i] = d.coord[gv];
+        s_coord[i] = xy;
+        if (F32G) {
+            s_p32[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+            s_c32[i] = make_float2((float)(xy.x - xref.x), (float)(xy.y - xref.y));
+        }
         if (i < NG) s_ais[i] = d.ais[gv];
         if (VISC) {
             double ml = sutherland_viscosity(c, p.x, p.w);
@@ -854,6 +866,40 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         }
         __syncthreads();                 // the gradients overwrite the staged terms
     }
+    if (F32G) {
+        // FP32 gradients for the FP32 reconstruction: per vertex its
+        // triangles' Green-Gauss terms in ascending order, from the FP32
+        // state and patch-relative coordinates (differences of nearby
+        // coordinates, so the normals keep their relative accuracy)
+        for (int g = t; g < NG; g += kBlock) {
+            const float ais = (float)s_ais[g];
+            float4 gx = make_float4(0.f, 0.f, 0.f, 0.f), gy = gx;
+            const int k1 = s_gptr[g + 1];
+            for (int k = s_gptr[g]; k < k1; ++k) {
+                const short4 tr = s_ltri[s_gtl[k] >> 2];
+                const float2 c1 = s_c32[tr.x], c2 = s_c32[tr.y], c3 = s_c32[tr.z];
+                const float4 a1 = s_p32[tr.x], a2 = s_p32[tr.y], a3 = s_p32[tr.z];
+                const float bx0 = c2.y - c3.y, bx1 = c3.y - c1.y, bx2 = c1.y - c2.y;
+                const float by0 = c3.x - c2.x, by1 = c1.x - c3.x, by2 = c2.x - c1.x;
+                gx.x = fmaf(a3.x, bx2, fmaf(a2.x, bx1, fmaf(a1.x, bx0, gx.x)));
+                gx.y = fmaf(a3.y, bx2, fmaf(a2.y, bx1, fmaf(a1.y, bx0, gx.y)));
+                gx.z = fmaf(a3.z, bx2, fmaf(a2.z, bx1, fmaf(a1.z, bx0, gx.z)));
+                gx.w = fmaf(a3.w, bx2, fmaf(a2.w, bx1, fmaf(a1.w, bx0, gx.w)));
+                gy.x = fmaf(a3.x, by2, fmaf(a2.x, by1, fmaf(a1.x, by0, gy.x)));
+                gy.y = fmaf(a3.y, by2, fmaf(a2.y, by1, fmaf(a1.y, by0, gy.y)));
+                gy.z = fmaf(a3.z, by2, fmaf(a2.z, by1, fmaf(a1.z, by0, gy.z)));
+                gy.w = fmaf(a3.w, by2, fmaf(a2.w, by1, fmaf(a1.w, by0, gy.w)));
+            }
+            const GT qx{gx.x * ais, gx.y * ais, gx.z * ais, gx.w * ais};
+            const GT qy{gy.x * ais, gy.y * ais, gy.z * ais, gy.w * ais};
+            s_gx[g] = qx;
+            s_gy[g] = qy;
+            if (MODE == 2 && g < NP) {
+                d.qx[v0 + g] = make_double4(qx.x, qx.y, qx.z, qx.w);
+                d.qy[v0 + g] = make_double4(qy.x, qy.y, qy.z, qy.w);
+            }
+        }
+    } else
     for (int g = t; g < NG; g += kBlock) {
         const double ais = s_ais[g];
         double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
@@ -914,8 +960,19 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             const V4f qy1{{(float)h1.x, (float)h1.y, (float)h1.z, (float)h1.w}};
             const V4f qx2{{(float)g2.x, (float)g2.y, (float)g2.z, (float)g2.w}};
             const V4f qy2{{(float)h2.x, (float)h2.y, (float)h2.z, (float)h2.w}};
-            const float dxf = (float)(x2.x - x1.x), dyf = (float)(x2.y - x1.y);
-            recon<LIM, float>(c, dxf, dyf, to_f(p1), to_f(p2), qx1, qy1, qx2, qy2, pl, pr);
+            float dxf, dyf;
+            V4f f1, f2;
+            if (F32G) {
+                const float2 c1 = s_c32[e.x], c2 = s_c32[e.y];
+                const float4 a1f = s_p32[e.x], a2f = s_p32[e.y];
+                dxf = c2.x - c1.x; dyf = c2.y - c1.y;
+                f1 = V4f{{a1f.x, a1f.y, a1f.z, a1f.w}};
+                f2 = V4f{{a2f.x, a2f.y, a2f.z, a2f.w}};
+            } else {
+                dxf = (float)(x2.x - x1.x); dyf = (float)(x2.y - x1.y);
+                f1 = to_f(p1); f2 = to_f(p2);
+            }
+            recon<LIM, float>(c, dxf, dyf, f1, f2, qx1, qy1, qx2, qy2, pl, pr);
             const EdgeFrame<float> efr{(float)fr.x, (float)fr.y, (float)fr.z};
             if (FLUX == 1) ok = roe_flux<float>(c, efr, pl, pr, ff);
             else if (FLUX == 2) kfvs_flux<float>(c, efr, pl, pr, ff);

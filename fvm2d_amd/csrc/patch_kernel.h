@@ -69,8 +69,10 @@ __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, i
     // terms per triangle touching P
     const int grad = NG * (split ? 32 : 64);
     const int grad_region = (((visc && NTP * 40 > grad) ? NTP * 40 : grad) + 15) & ~15;
-    return NLa * 48 + NLa * 8 * (flow == 3 ? 4 : (visc ? 2 : 0)) + grad_region + pre + work +
-           ((index + 15) & ~15);
+    // per local vertex: state and coordinates; with viscosity 2 (SA: 4)
+    // doubles; inviscid split precision an FP32 state and coordinates (24 B)
+    return NLa * 48 + NLa * 8 * (flow == 3 ? 4 : (visc ? 2 : (split ? 3 : 0))) + grad_region + pre +
+           work + ((index + 15) & ~15);
 }
 
 // bytes of the LDS carve-up of the SA step over one patch: state,
