@@ -450,7 +450,7 @@ def main() -> int:
             e1 = time.perf_counter() - t1
         split = {"value": round(npt * a.steps / e1, 1), "unit": "vertex-iterations/s",
                  "dcl": abs(float(h1[-1, 4] - hist[-1, 4])), "dcd": abs(float(h1[-1, 5] - hist[-1, 5])),
-                 "note": "same workload and step count with FP32 reconstruction and flux; "
+                 "note": "same workload and step count with FP32 gradients, reconstruction and flux; "
                          "dcl, dcd against this FP64 run"}
     batched = None if a.no_batch else batch_legs(g, s.geo, kw, value, a.warmup)
     served = None if a.no_serving else serving_leg()
