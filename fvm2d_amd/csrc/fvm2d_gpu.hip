@@ -1136,7 +1136,16 @@ __global__ void __launch_bounds__(kBlock) k_wall_dist(int np, const double2 *__r
     if (i < np) wd[i] = solid[i] ? 0.0 : w;
 }
 
-__global__ void k_begin_chunk(DevState *S) { S->nhist = 0; }
+// Start of an iterate call: history rows from zero, and no first-stage error
+// pending (one is moved to err_code within its own iteration, so what is
+// left here came from a stage that ran without its iteration: the stage
+// timing, or a first stage beside the monitors that stopped the solve).
+__global__ void k_begin_chunk(DevState *S)
+{
+    S->nhist = 0;
+    S->err1_code = 0;
+    S->err1_index = INT_MAX;
+}
 
 // plain copy for the achievable-bandwidth figure: 16 bytes per thread and
 // step, grid-stride
@@ -1396,7 +1405,11 @@ __global__ void __launch_bounds__(kBlock) k_vortex_batch(const BatchMember *__re
 __global__ void k_begin_batch(const BatchMember *__restrict__ mem, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) mem[i].S->nhist = 0;
+    if (i < n) {                        // as k_begin_chunk, per member
+        mem[i].S->nhist = 0;
+        mem[i].S->err1_code = 0;
+        mem[i].S->err1_index = INT_MAX;
+    }
 }
 
 __global__ void k_gather_batch(const BatchMember *__restrict__ mem, int n, DevState *out)
