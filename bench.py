@@ -317,6 +317,55 @@ def config_legs(iters_lam: int, iters_rans: int) -> dict:
     return out
 
 
+def rans_converge(max_iters: int, chunk: int = 1000) -> dict:
+    """The RANS workload as specified: the batch of 8 SA solves on the
+    2049x513 grid (alpha 0..14 deg) run to a 1e-8 drop of the density
+    residual or max_iters iterations, in FP64 and in split precision; per
+    solve the iterations, status, final residual drop, CL and CD, and the
+    split-minus-FP64 differences of CL and CD at the end."""
+    from fvm2d_amd import mesh
+    from fvm2d_amd.solver import Batch, Inputs, Solver, geometric
+    g = mesh.naca0012_2049x513()
+    geo = geometric(g.coord, g.elem, g.ptype, iflow=3, wall_distance="gpu")
+    kw = dict(iflow=3, iflux=1, ilimit=1, mach_inf=0.15, rey=6.0e6, cfl=1.0, maxiter=max_iters,
+              minres=1e-8, nut_factor=3.0)
+    alphas = [2.0 * k for k in range(8)]
+    res = {}
+    for prec in (0, 1):
+        sol = [Solver(g, Inputs.from_kwargs(precision=prec, aoa_deg=al, **kw), geometry=geo)
+               for al in alphas]
+        t0 = time.perf_counter()
+        with Batch() as b:
+            for x in sol:
+                b.add(x)
+            done = 0
+            while done < max_iters:
+                st = b.iterate(min(chunk, max_iters - done))
+                done += chunk
+                mons = [x.monitor()[0] for x in sol]
+                print(f"RANS batch, precision {prec}: {min(done, max_iters)} iterations, "
+                      f"fres {max(m['fres'] for m in mons):.3e}, status {st}",
+                      file=sys.stderr, flush=True)
+                if all(v != 0 for v in st):
+                    break
+            mons = [x.monitor()[0] for x in sol]
+        wall = time.perf_counter() - t0
+        for x in sol:
+            x.close()
+        res[prec] = (mons, st, wall)
+    m0, m1 = res[0][0], res[1][0]
+    return {"workload": "NACA0012 2049x513 (1 050 624 vertices), SA, M 0.15, Re 6e6, "
+                        "alpha 0..14 deg, a batch of 8, to a 1e-8 drop or %d iterations" % max_iters,
+            "alphas_deg": alphas,
+            "iterations_fp64": [m["iter"] for m in m0], "iterations_split": [m["iter"] for m in m1],
+            "status_fp64": res[0][1], "status_split": res[1][1],
+            "fres_fp64": [m["fres"] for m in m0], "fres_split": [m["fres"] for m in m1],
+            "cl_fp64": [m["cl"] for m in m0], "cd_fp64": [m["cd"] for m in m0],
+            "dcl": [abs(a["cl"] - b["cl"]) for a, b in zip(m0, m1)],
+            "dcd": [abs(a["cd"] - b["cd"]) for a, b in zip(m0, m1)],
+            "wall_s_fp64": round(res[0][2], 1), "wall_s_split": round(res[1][2], 1)}
+
+
 def main() -> int:
     faulthandler.enable()
     ap = argparse.ArgumentParser()
@@ -336,6 +385,9 @@ def main() -> int:
     ap.add_argument("--configs", nargs=2, type=int, default=None, metavar=("LAM", "RANS"),
                     help="only run the split-vs-FP64 legs of the laminar 1025x257 and RANS "
                          "2049x513 batch-of-8 workloads for LAM and RANS iterations")
+    ap.add_argument("--rans-converge", type=int, default=0, metavar="N",
+                    help="only run the RANS batch of 8 to a 1e-8 residual drop or N iterations, "
+                         "FP64 and split, and report CL/CD and their differences")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES, five times: the "
                          "reference reading for the FETCH_SIZE/WRITE_SIZE counters")
@@ -352,6 +404,9 @@ def main() -> int:
         return 0
     if a.configs:
         print(json.dumps(config_legs(*a.configs)), flush=True)
+        return 0
+    if a.rans_converge:
+        print(json.dumps(rans_converge(a.rans_converge)), flush=True)
         return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the workload is a single-GPU solve and this build "
