@@ -133,8 +133,9 @@ def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
     in one launch per RK stage, FP64 and split precision; (b) 48 requests of
     the serving workload's distribution (NACA 4-digit airfoils, 1k-20k
     vertices, 70 % Euler / 30 % SA-RANS, seeded) as two batches (one per flow
-    kind, one stream each), split precision, against the same requests
-    solved one after another by the plain FP64 path."""
+    kind, one stream each) with the serving precisions (Euler split, SA-RANS
+    FP64), against the same requests solved one after another by the plain
+    FP64 path."""
     from fvm2d_amd import serving
     from fvm2d_amd.solver import Batch, Inputs, Solver
     out = {}
@@ -176,7 +177,8 @@ def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
             tp += time.perf_counter() - t0
             vi += x.np * (x.monitor()[0]["iter"] - i0)
     plain = vi / tp
-    # batched split precision, one batch per flow kind, chunks launched together
+    # batched with the serving precisions, one batch per flow kind, chunks
+    # launched together
     sol = {"euler": [], "rans": []}
     for q, g1, geo1 in prepared:
         sol[q.flow].append(serving.create(q, g1, geo1, cfg))
@@ -207,7 +209,8 @@ def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
     out["mixed_48_requests"] = {
         "workload": "48 NACA 4-digit requests, 1k-20k vertices log-uniform, 70/30 Euler/SA-RANS, "
                     f"{nv} vertices in all, {steps} iterations each",
-        "batched_split": round(vb / dt, 1), "plain_fp64_one_at_a_time": round(plain, 1),
+        "precision": "Euler split, SA-RANS FP64",
+        "batched": round(vb / dt, 1), "plain_fp64_one_at_a_time": round(plain, 1),
         "speedup": round(vb / dt / plain, 2)}
     return out
 
@@ -216,7 +219,7 @@ def serving_leg(n: int = 32, seed: int = 11, max_iters: int = 300) -> dict:
     """Solve latency of the serving loop (fvm2d_amd/serving.py) on a bounded
     stream: n seeded requests of 600-6000 vertices (NACA 4-digit airfoils,
     70/30 Euler/SA-RANS), each to a 1e-6 residual drop or max_iters
-    iterations, in split precision.  Offered at 400 requests/s (faster than
+    iterations, Euler in split precision and SA-RANS in FP64.  Offered at 400 requests/s (faster than
     the loop serves them), the run gives the saturation rate in solves/s; then the same requests arrive as Poisson
     processes at 50 % and 90 % of that rate.  A latency runs from a request's
     arrival to its result: host meshing and preprocessing, admission, the
@@ -246,7 +249,8 @@ def serving_leg(n: int = 32, seed: int = 11, max_iters: int = 300) -> dict:
     sat = run(400.0)
     rate = sat["solves_per_s"]
     return {"workload": f"{n} requests, NACA 4-digit O-grids of 600-6000 vertices, 70/30 "
-                        f"Euler/SA-RANS, split precision, residual drop 1e-6 or {max_iters} iterations",
+                        f"Euler/SA-RANS (Euler split precision, SA-RANS FP64), residual drop 1e-6 or "
+                        f"{max_iters} iterations",
             "saturation": sat, "load_50": run(0.5 * rate), "load_90": run(0.9 * rate)}
 
 

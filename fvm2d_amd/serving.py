@@ -116,7 +116,13 @@ def poisson_arrivals(reqs: list[Request], rate: float, seed: int) -> list[Reques
 @dataclasses.dataclass
 class ServeConfig:
     hbm_budget: int = 64 << 30        # bytes of context state admitted at once
-    precision: int = 1                # 1: split precision (FP32 flux), 0: FP64
+    precision: int = 1                # Euler requests: 1 split precision (FP32 flux), 0 FP64
+    # SA-RANS requests: FP64.  Split precision tracks FP64 on short runs
+    # (|dCL| ~ 1e-6 after 200 iterations) but not to convergence: on the
+    # 2049x513 RANS batch its residual stalls at the level of the first
+    # iteration's while FP64 drops to 1e-3, and CL differs by up to 5e-2
+    # after 20 000 iterations.
+    rans_precision: int = 0
     minres: float = 1e-6              # residual drop that ends a solve
     max_iters: int = 2000             # iteration cap of a solve
     step_vertex_iters: float = 2e7     # work of one engine step per class (sets its chunk)
@@ -130,11 +136,15 @@ class ServeConfig:
     patch_size: int = 64
 
 
+def precision_for(req: Request, cfg: ServeConfig) -> int:
+    return cfg.rans_precision if req.flow == "rans" else cfg.precision
+
+
 def inputs_for(req: Request, cfg: ServeConfig) -> Inputs:
     return Inputs(iflow=3 if req.flow == "rans" else 1, iflux=1, ilimit=1,
                   mach_inf=req.mach, aoa_deg=req.aoa, rey=req.rey, cfl=cfg.cfl,
                   iterlast=0, maxiter=cfg.max_iters, minres=cfg.minres,
-                  saveinterval=0, precision=cfg.precision, nut_factor=3.0, sa_compat=0,
+                  saveinterval=0, precision=precision_for(req, cfg), nut_factor=3.0, sa_compat=0,
                   patch_size=cfg.patch_size)
 
 
@@ -161,7 +171,7 @@ def footprint(geo, req: Request, cfg: ServeConfig) -> int:
     d = geo.dims
     return int(gpu.lib().fvm_gpu_footprint_estimate(d["np"], d["nt"], d["netot"],
                                                       3 if req.flow == "rans" else 1,
-                                                      cfg.precision))
+                                                      precision_for(req, cfg)))
 
 
 class Server:
@@ -312,7 +322,8 @@ def summary(results: list[Result], wall: float) -> dict:
 
 def solve_alone(req: Request, cfg: ServeConfig, precision: int | None = None):
     """The same request solved by itself with the plain path (fvm_gpu_iterate)."""
-    c = dataclasses.replace(cfg, precision=cfg.precision if precision is None else precision)
+    c = cfg if precision is None else dataclasses.replace(cfg, precision=precision,
+                                                          rans_precision=precision)
     g, geo = prepare(req, c)
     with create(req, g, geo, c) as s:
         try:
