@@ -370,6 +370,37 @@ def rans_converge(max_iters: int, chunk: int = 1000) -> dict:
             "wall_s_fp64": round(res[0][2], 1), "wall_s_split": round(res[1][2], 1)}
 
 
+def lam_converge(iters: int, chunk: int = 1000) -> dict:
+    """The laminar workload (1025x257, M 0.5, Re 5000) run for `iters`
+    iterations in FP64 and in split precision; residual drop, CL, CD and the
+    split-minus-FP64 differences every `chunk` iterations."""
+    import numpy as np
+    from fvm2d_amd import mesh
+    from fvm2d_amd.solver import Inputs, Solver, geometric
+    g = mesh.naca0012_1025x257()
+    geo = geometric(g.coord, g.elem, g.ptype, iflow=2)
+    kw = dict(iflow=2, iflux=1, ilimit=1, mach_inf=0.5, aoa_deg=0.0, rey=5000.0, cfl=1.0,
+              maxiter=10**9, minres=0.0)
+    hist = {}
+    for prec in (0, 1):
+        with Solver(g, Inputs.from_kwargs(precision=prec, **kw), geometry=geo) as x:
+            parts = []
+            for k in range(0, iters, chunk):
+                h, _ = x.iterate(min(chunk, iters - k))
+                parts.append(h)
+                print(f"laminar, precision {prec}: {k + h.shape[0]} iterations, fres {h[-1, 1]:.3e}",
+                      file=sys.stderr, flush=True)
+            hist[prec] = np.concatenate(parts)
+    h0, h1 = hist[0], hist[1]
+    rows = []
+    for it in range(chunk - 1, min(h0.shape[0], h1.shape[0]), chunk):
+        rows.append({"iter": int(h0[it, 0]), "fres_fp64": float(h0[it, 1]), "fres_split": float(h1[it, 1]),
+                     "cl_fp64": float(h0[it, 4]), "cd_fp64": float(h0[it, 5]),
+                     "dcl": abs(float(h1[it, 4] - h0[it, 4])), "dcd": abs(float(h1[it, 5] - h0[it, 5]))})
+    return {"workload": "NACA0012 1025x257 (263 168 vertices), laminar, M 0.5, Re 5000, FP64 and split",
+            "rows": rows}
+
+
 def main() -> int:
     faulthandler.enable()
     ap = argparse.ArgumentParser()
@@ -389,6 +420,9 @@ def main() -> int:
     ap.add_argument("--configs", nargs=2, type=int, default=None, metavar=("LAM", "RANS"),
                     help="only run the split-vs-FP64 legs of the laminar 1025x257 and RANS "
                          "2049x513 batch-of-8 workloads for LAM and RANS iterations")
+    ap.add_argument("--lam-converge", type=int, default=0, metavar="N",
+                    help="only run the laminar 1025x257 workload N iterations, FP64 and split, "
+                         "and report the differences every 1000 iterations")
     ap.add_argument("--rans-converge", type=int, default=0, metavar="N",
                     help="only run the RANS batch of 8 to a 1e-8 residual drop or N iterations, "
                          "FP64 and split, and report CL/CD and their differences")
@@ -411,6 +445,9 @@ def main() -> int:
         return 0
     if a.rans_converge:
         print(json.dumps(rans_converge(a.rans_converge)), flush=True)
+        return 0
+    if a.lam_converge:
+        print(json.dumps(lam_converge(a.lam_converge)), flush=True)
         return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the workload is a single-GPU solve and this build "
