@@ -130,3 +130,16 @@ def test_engine_timeout(model):
     cfg = serving.ServeConfig(max_iters=10**9, step_vertex_iters=1e3)
     with pytest.raises(TimeoutError):
         serving.Server(cfg).run(reqs, timeout=0.5)
+
+
+def test_precision_per_flow_kind():
+    """Euler requests are served in split precision, SA-RANS in FP64 (split
+    does not follow FP64 to convergence on SA-RANS); both can be set."""
+    reqs = serving.workload(40, 3)
+    cfg = serving.ServeConfig()
+    for q in reqs:
+        want = 0 if q.flow == "rans" else 1
+        assert serving.precision_for(q, cfg) == want
+        assert serving.inputs_for(q, cfg).precision == want
+    cfg2 = serving.ServeConfig(precision=0, rans_precision=1)
+    assert {serving.precision_for(q, cfg2) for q in reqs} == {0, 1}
