@@ -117,11 +117,11 @@ def poisson_arrivals(reqs: list[Request], rate: float, seed: int) -> list[Reques
 class ServeConfig:
     hbm_budget: int = 64 << 30        # bytes of context state admitted at once
     precision: int = 1                # Euler requests: 1 split precision (FP32 flux), 0 FP64
-    # SA-RANS requests: FP64.  Split precision tracks FP64 on short runs
-    # (|dCL| ~ 1e-6 after 200 iterations) but not to convergence: on the
-    # 2049x513 RANS batch its residual stalls at the level of the first
-    # iteration's while FP64 drops to 1e-3, and CL differs by up to 5e-2
-    # after 20 000 iterations.
+    # SA-RANS requests: FP64.  With FP32 fluxes at low Mach on wall-resolved
+    # grids the residual norm sits at a noise floor (a residual-drop stop
+    # never triggers) and the solution leaves the FP64 one during the
+    # unsteady start (CL up to 5e-2 apart after 20 000 iterations on the
+    # 2049x513 RANS batch); Euler solves follow FP64 in split precision.
     rans_precision: int = 0
     minres: float = 1e-6              # residual drop that ends a solve
     max_iters: int = 2000             # iteration cap of a solve
