@@ -1556,7 +1556,7 @@ void set_consts(fvm_ctx *x)
 
 inline unsigned nblk(int n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-// host copy of the free-stream Sutherland viscosity (src/viscosity.f:245-258)
+// host copy of the free-stream Sutherland viscosity (src/viscosity.f:42-55)
 double sutherland_host(const Consts &c, double rho, double p)
 {
     double temp = p / rho / 1.0;
@@ -2009,7 +2009,7 @@ void drop_graphs(fvm_ctx *x)
 }
 
 
-// prim, nut in the reference layout; mu/mul follow (src/initialize.f:187-207)
+// prim, nut in the reference layout; mu/mul follow (src/initialize.f:60-103)
 // caller order <-> device order (vertices renumbered by patch)
 std::vector<double> to_device(const fvm_ctx *x, const double *a, int comps)
 {
@@ -2418,8 +2418,8 @@ int validate_layout(const fvm_mesh *m, const Layout &L)
 
 extern "C" {
 
-// Restart (src/initialize.f:91-103): that path leaves nut_inf at 0
-// (src/initialize.f:153), which the farfield inflow of nu_tilde uses.
+// Restart (src/initialize.f:91-103): that path never sets nut_inf, which
+// keeps the zero of its COMMON block; the farfield inflow of nu_tilde uses it.
 int fvm_gpu_set_state(fvm_ctx *x, const double *prim, const double *nut)
 {
     x->c.nut_inf = 0.0;

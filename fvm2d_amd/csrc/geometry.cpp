@@ -59,7 +59,7 @@ extern "C" int fvm_geometric(int np, int nt, const double *coord, int *elem, con
     const Pt *P = reinterpret_cast<const Pt *>(coord);      // P[p-1]
     auto V = [&](int t, int k) -> int & { return elem[3 * (t - 1) + k]; };
 
-    // point classes (read_grid.f:175-207)
+    // point classes (src/read_grid.f:62-94)
     int nsp = 0, nfp = 0;
     for (int p = 1; p <= np; ++p) {
         if (ptype[p - 1] == kSolid) spts[nsp++] = p;

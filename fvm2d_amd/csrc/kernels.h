@@ -394,7 +394,7 @@ __device__ __forceinline__ double pow_1_5(double x)
     return hi + (lo + x * e);
 }
 
-// Sutherland viscosity, src/viscosity.f:245-258 (the reference's
+// Sutherland viscosity, src/viscosity.f:42-55 (the reference's
 // (temp/T_inf)**1.5, correctly rounded)
 __device__ __forceinline__ double sutherland_viscosity(const Consts &c, double rho, double p)
 {
@@ -404,7 +404,7 @@ __device__ __forceinline__ double sutherland_viscosity(const Consts &c, double r
     return pow_1_5(temp / c.T_inf) * (num / den) / c.rey;
 }
 
-// SA eddy viscosity, src/viscosity.f:261-274
+// SA eddy viscosity, src/viscosity.f:58-71
 __device__ __forceinline__ double mu_turb(const Consts &c, double mu_lam, double rho, double nu)
 {
     double chi = nu * rho / mu_lam;

@@ -60,7 +60,7 @@ typedef struct {
                            (<= 256; 0 = automatic: whole rounds of resident
                            workgroups, about 64 vertices a patch)               */
     double nut_factor;  /* initial nu_tilde = nut_factor*mu_l/rho
-                           (src/initialize.f:190 uses 0.1)                       */
+                           (src/initialize.f:71 uses 0.1)                       */
 } fvm_params;
 
 /* Mesh arrays as geometric() leaves them (src/geometric.f:15-60). */
@@ -86,7 +86,7 @@ typedef struct fvm_ctx fvm_ctx;
 
 /* Replaces geometric(coord,elarea,cvarea,ds,dsb,mcarea,wd,drmin)
  * (src/geometric.f:2-63) and read_grid's point classification
- * (src/read_grid.f:175-207).  Host code.  elem is re-oriented in place
+ * (src/read_grid.f:62-94).  Host code.  elem is re-oriented in place
  * (src/tri_orient.f).  Edge arrays hold max_edges columns (3*nt suffices).
  * wd/wd1 are written only for iflow == 3.
  * dims: np nt ne netot nsp nfp nbe nswe1 nswe2 nffe1 nffe2 err_index. */

@@ -499,7 +499,7 @@ static void green_gauss(orc_ctx *c, int visc)      /* green_gauss.f:2-92,97-194 
             }
         }
         if (!visc) continue;
-        /* tri_visc, green_gauss.f:223-285 */
+        /* tri_visc, green_gauss.f:132-194 */
         double en[3], ext, eyt, um, vm, mut, mult, prim5t, rhot, chi, chi3, fv1;
         double txx, txy, tyy, gpr, efx, efy, areat;
         for (int m = 0; m < 3; m++) en[m] = PRIM(4, n[m]) / PRIM(1, n[m]);
@@ -858,7 +858,7 @@ void orc_update(orc_ctx *c, int irk)
             c->con_new[4 * (j - 1) + 1] = 0.0;
             c->con_new[4 * (j - 1) + 2] = 0.0;
         }
-    for (int p = 1; p <= c->np; p++) {              /* con_to_prim, convert.f:124-140 */
+    for (int p = 1; p <= c->np; p++) {              /* con_to_prim, convert.f:31-47 */
         const double *u = &c->con_new[4 * (p - 1)];
         PRIM(1, p) = u[0];
         PRIM(2, p) = u[1] / u[0];
@@ -1083,7 +1083,7 @@ orc_ctx *orc_create(int np, int nt, const double *coord, const int *elem,
 
     int rc = ORC_OK;
     math_constants(c);
-    /* read_grid.f:175-207: point classification */
+    /* read_grid.f:62-94: point classification */
     for (int ip = 1; ip <= np; ip++) {
         int t = V1(c->ptype, ip);
         if (t == SOLID) V1(c->spts, ++c->nsp) = ip;

@@ -26,7 +26,7 @@ typedef struct {
     int cell_type;       /* 1 median, 2 Barth                                */
     int sa_compat;       /* 1: keep sa_model.f:57-62 and wall_dist.f:46 as shipped */
     double mach_inf, aoa_deg, rey, cfl, minres, xref, yref;
-    double nut_factor;   /* initial nu_tilde = nut_factor*mul/rho (initialize.f:190 uses 0.1) */
+    double nut_factor;   /* initial nu_tilde = nut_factor*mul/rho (initialize.f:71 uses 0.1) */
 } orc_input;
 
 typedef struct orc_ctx orc_ctx;
