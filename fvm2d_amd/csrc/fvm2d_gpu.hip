@@ -63,6 +63,10 @@ struct DevState {
     // first-stage errors, kept apart from err_code until the second stage
     // (the monitors of the previous iteration may run beside the first stage)
     int err1_code, err1_index;
+    // lowest vertex failing the positivity check in a last stage; the
+    // monitors turn it into err_code unless an edge of the same stage failed
+    // first (the reference stops in the flux, before the update runs)
+    int errv_index;
 };
 
 struct Dev {
@@ -546,7 +550,7 @@ __device__ __forceinline__ void final_body(const Dev &d, const Consts &c, DevSta
     // partial and the first solid edge of each thread) is issued before any
     // is used, so the workgroup waits for memory once; only the writes
     // depend on whether the solve is still running.
-    const int done = S->done, err = S->err_code;
+    const int done = S->done, err0 = S->err_code, errv = S->errv_index;
     const int iter = S->iter, nhist = S->nhist;
     const double fres1 = S->fres1;
     const int t = threadIdx.x;
@@ -571,6 +575,12 @@ __device__ __forceinline__ void final_body(const Dev &d, const Consts &c, DevSta
     double2 f = make_double2(xf, yf);
     block_reduce_final<kFinal>(acc, f);
     if (done || t != 0) return;
+    int err = err0;
+    if (!err && errv != INT_MAX) {
+        err = FVM_E_NONPOSITIVE;
+        S->err_index = errv;
+        S->err_code = err;
+    }
     if (err) { S->status = err; S->done = 1; return; }
     const Red tot = acc;
     double qdyn = 0.5 * c.r_inf * sq(c.q_inf);
@@ -1059,9 +1069,8 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
                 if (dif > 0.0) { red.m = dif; red.i = d.orig[v]; }
                 if (!TURB) {
                     if (q.x <= 0.0 || q.w <= 0.0) {
-                        atomicMin(&S->err_index, d.orig[v]);
+                        atomicMin(&S->errv_index, d.orig[v]);
                         S->err_stage = irk;
-                        S->err_code = FVM_E_NONPOSITIVE;
                     }
                     // time_step of the next iteration (src/time_step.f:13-28)
                     double qq = sqrt(sq(q.y) + sq(q.z));
@@ -1422,7 +1431,13 @@ __global__ void k_gather_batch(const BatchMember *__restrict__ mem, int n, DevSt
 
 // ----------------------------------------------------------------- host side
 
+// Handles carry a tag that fvm_gpu_create / fvm_batch_create set and destroy
+// clears: a stale or foreign handle is refused (FVM_E_INPUT) instead of being
+// dereferenced.
+constexpr unsigned kCtxTag = 0x46564d43u, kBatchTag = 0x46564d42u;
+
 struct fvm_ctx {
+    unsigned tag = kCtxTag;
     int cus = 256, per_cu = 4, lds_per_cu = 160 * 1024;   // device and occupancy
     int target = 0;                                         // patch size asked of the partition
     int max_patch = 0;
@@ -1458,6 +1473,7 @@ struct fvm_ctx {
 // Many solves of one kind sharing each launch (see BatchMember).  Membership
 // changes rebuild the work lists and the captured iterations.
 struct fvm_batch {
+    unsigned tag = kBatchTag;
     hipStream_t stream = nullptr;
     std::vector<fvm_ctx *> mem;
     BatchMember *d_mem = nullptr;
@@ -1469,7 +1485,11 @@ struct fvm_batch {
     hipGraphExec_t graph_n = nullptr, graph_1 = nullptr;
     bool graphs_ok = true;
     bool pending = false;             // a launched chunk not waited for yet
-    std::vector<DevState> hs;
+    // members' monitors after a launched chunk: pinned host memory, so the
+    // copy is ordered on the batch's stream and fvm_batch_launch returns at
+    // once (a copy into pageable memory may complete only with the chunk)
+    DevState *hs = nullptr;
+    int hs_cap = 0;
 };
 
 namespace {
@@ -1483,6 +1503,36 @@ namespace {
             return FVM_E_HIP;                                                       \
         }                                                                           \
     } while (0)
+
+// As HIPCHK, where copies between host buffers of the caller's frame and the
+// device may still be queued on `st`: the stream drains before the return,
+// so no copy outlives the buffers it reads or writes.
+#define HIPCHK_S(st, x)                                                             \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess) {                                                     \
+            std::fprintf(stderr, "fvm2d_gpu: %s failed: %s (%s:%d)\n", #x,          \
+                         hipGetErrorString(e_), __FILE__, __LINE__);                \
+            (void)hipStreamSynchronize(st);                                         \
+            return FVM_E_HIP;                                                       \
+        }                                                                           \
+    } while (0)
+
+inline bool ctx_ok(const fvm_ctx *x) { return x && x->tag == kCtxTag; }
+}  // namespace
+int settle(fvm_ctx *x);
+namespace {
+inline bool batch_ok(const fvm_batch *b) { return b && b->tag == kBatchTag; }
+
+// monitor state with no error pending
+inline DevState no_errors()
+{
+    DevState s{};
+    s.err_index = INT_MAX;
+    s.err1_index = INT_MAX;
+    s.errv_index = INT_MAX;
+    return s;
+}
 
 template <class T>
 int dalloc(fvm_ctx *x, T **p, size_t n)
@@ -1945,44 +1995,70 @@ int configure(fvm_ctx *x, int variant, int vb, int xcd)
         if (rc) return rc;
     }
     if (x->batch) x->batch->dirty = true;
+    HIPCHK(hipStreamSynchronize(x->stream));      // no replay of the old graphs in flight
     if (x->graph_n) (void)hipGraphExecDestroy(x->graph_n);
     if (x->graph_1) (void)hipGraphExecDestroy(x->graph_1);
     x->graph_n = x->graph_1 = nullptr;
+    x->graphs_ok = true;
     launch_tstep(x);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));
     return FVM_OK;
 }
 
+// Ends a capture on `st` and instantiates it into *out.  On any failure the
+// capture is ended all the same (the stream is left out of capture mode),
+// what was built is destroyed, *out stays null and the sticky error is
+// cleared, so the caller can fall back to launching the kernels directly.
+int end_capture(hipStream_t st, hipGraphExec_t *out)
+{
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(st, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+    }
+    hipGraphExec_t ex = nullptr;
+    if (e == hipSuccess && g) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    else if (e == hipSuccess) e = hipErrorInvalidValue;
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        if (ex) (void)hipGraphExecDestroy(ex);
+        (void)hipGetLastError();
+        return FVM_E_HIP;
+    }
+    *out = ex;
+    return FVM_OK;
+}
+
 int build_graph(fvm_ctx *x, int iters, hipGraphExec_t *out)
 {
-    hipGraph_t g = nullptr;
-    if (hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return FVM_E_HIP;
+    *out = nullptr;
+    if (hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return FVM_E_HIP;
+    }
     const bool merge = x->variant == 3 && monitors_merge(x->par, x->c.vortex == 1 && x->c.mach_inf < 1.0 && x->nff > 0);
     for (int i = 0; i < iters; ++i) one_iteration(x, nullptr, merge && i > 0, !merge || i == iters - 1);
-    if (hipStreamEndCapture(x->stream, &g) != hipSuccess) return FVM_E_HIP;
-    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    return e == hipSuccess ? FVM_OK : FVM_E_HIP;
+    return end_capture(x->stream, out);
 }
 
 int reset_state(fvm_ctx *x)
 {
-    DevState s{};
+    DevState s = no_errors();
     s.iter = x->par.iterlast;
     s.fres = x->par.minres + 1.0;
-    s.err_index = INT_MAX;
-    s.err1_index = INT_MAX;
     if (!(x->par.maxiter > 0)) { s.done = 1; s.status = FVM_ITER_LIMIT; }
     x->hS = s;
-    HIPCHK(hipMemcpyAsync(x->S, &s, sizeof s, hipMemcpyHostToDevice, x->stream));
-    DevState h{};
-    h.err_index = INT_MAX;
-    h.err1_index = INT_MAX;
-    HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
+    const DevState h = no_errors();
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->S, &s, sizeof s, hipMemcpyHostToDevice, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
     if (x->nff > 0)       // COMMON cl starts at zero
         hipLaunchKernelGGL(k_farfield_state, dim3(nblk(x->nff)), dim3(kBlock), 0, x->stream, x->d, x->c, 0.0);
-    HIPCHK(hipGetLastError());
+    HIPCHK_S(x->stream, hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));
     return FVM_OK;
 }
@@ -2033,7 +2109,7 @@ int load_state(fvm_ctx *x, const double *prim_caller, const double *nut_caller)
     if (nut_caller) nutv = to_device(x, nut_caller, 1);
     const double *prim = primv.data();
     const double *nut = nut_caller ? nutv.data() : nullptr;
-    HIPCHK(hipMemcpyAsync(x->d.prim, prim, sizeof(double) * 4 * np, hipMemcpyHostToDevice, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->d.prim, prim, sizeof(double) * 4 * np, hipMemcpyHostToDevice, x->stream));
     std::vector<double> nu(np, 0.0), mul(np, 0.0), mu(np, 0.0);
     if (x->par.iflow != 1) {
         for (int p = 0; p < np; ++p) {
@@ -2050,9 +2126,9 @@ int load_state(fvm_ctx *x, const double *prim_caller, const double *nut_caller)
             }
         }
     }
-    HIPCHK(hipMemcpyAsync(x->d.nut, nu.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
-    HIPCHK(hipMemcpyAsync(x->d.mul, mul.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
-    HIPCHK(hipMemcpyAsync(x->d.mu, mu.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->d.nut, nu.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->d.mul, mul.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->d.mu, mu.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     int rc = reset_state(x);
     if (rc) return rc;
@@ -2352,13 +2428,10 @@ int validate_layout(const fvm_mesh *m, const Layout &L)
     const PatchHost &ph = L.ph;
     const int npatch = (int)L.patch_ptr.size() - 1;
     if ((int)ph.hdr.size() != kPatchHdr * npatch || L.patch_ptr.back() != np) return 7;
-    const int tw = 8;
-    (void)tw;
     for (int k = 0; k < npatch; ++k) {
         const int *h = &ph.hdr[kPatchHdr * k];
         const int v0 = h[PH_V0], NP = h[PH_NP], NL = h[PH_NL], NG = h[PH_NG];
         const int NT = h[PH_NTRI], NE = h[PH_NEDGE], NB = h[PH_NBND];
-    const int NGTL = h[PH_NGTL], NPSL = h[PH_NPSL];
         if (h[PH_NGTL] != ph.gptr[h[PH_GPTR] + NG] || h[PH_NPSL] != ph.pptr[h[PH_PPTR] + NP]) return 25;
         if (v0 != L.patch_ptr[k] || NP != L.patch_ptr[k + 1] - v0 || NP < 1 || NP > kBlock) return 8;
         if (NG < NP || NL < NG || h[PH_HALO] + (NL - NP) > (int)ph.halo.size()) return 9;
@@ -2422,7 +2495,10 @@ extern "C" {
 // keeps the zero of its COMMON block; the farfield inflow of nu_tilde uses it.
 int fvm_gpu_set_state(fvm_ctx *x, const double *prim, const double *nut)
 {
+    if (!ctx_ok(x) || !prim) return FVM_E_INPUT;
+    if (int rc = settle(x)) return rc;
     x->c.nut_inf = 0.0;
+    HIPCHK(hipStreamSynchronize(x->stream));
     drop_graphs(x);
     return load_state(x, prim, nut);
 }
@@ -2494,6 +2570,7 @@ int fvm_patch_plan(const fvm_mesh *m, int iflow, int precision, int patch_size, 
 
 int fvm_gpu_layout_info(const fvm_ctx *x, long long *info)
 {
+    if (!ctx_ok(x) || !info) return FVM_E_INPUT;
     info[0] = x->pa.npatch;
     info[1] = x->max_patch;
     info[2] = x->pa.lds_bytes;
@@ -2505,7 +2582,7 @@ int fvm_gpu_layout_info(const fvm_ctx *x, long long *info)
     return FVM_OK;
 }
 
-long long fvm_gpu_footprint(const fvm_ctx *x) { return x ? x->bytes : 0; }
+long long fvm_gpu_footprint(const fvm_ctx *x) { return ctx_ok(x) ? x->bytes : 0; }
 
 int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
 {
@@ -2638,7 +2715,8 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     TRY(dalloc(x, &d.bflux, 2 * (size_t)std::max(nb, 1)));
     TRY(dalloc(x, &d.uout, (size_t)std::max(x->nff, 1)));
     TRY(dalloc(x, &d.wall, 2 * (size_t)std::max(x->nsw, 1)));
-    TRY(hipMemset(d.wall, 0, sizeof(double4) * 2 * std::max(x->nsw, 1)) == hipSuccess ? FVM_OK : FVM_E_HIP);
+    TRY(hipMemsetAsync(d.wall, 0, sizeof(double4) * 2 * std::max(x->nsw, 1), x->stream) == hipSuccess ? FVM_OK
+                                                                                                   : FVM_E_HIP);
     {
         std::vector<double2> ws(std::max(x->nsw, 1), make_double2(0.0, 0.0));
         for (int b = 0; b < x->nsw; ++b) {
@@ -2707,11 +2785,13 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
 int fvm_gpu_destroy(fvm_ctx *x)
 {
     if (!x) return FVM_OK;
+    if (x->tag != kCtxTag) return FVM_E_INPUT;
     if (x->batch) (void)fvm_batch_remove(x->batch, x);
     if (x->stream) (void)hipStreamSynchronize(x->stream);
     drop_graphs(x);
     for (void *p : x->allocs) (void)hipFree(p);
     if (x->stream) (void)hipStreamDestroy(x->stream);
+    x->tag = 0;
     delete x;
     return FVM_OK;
 }
@@ -2781,41 +2861,45 @@ int fvm_gpu_wall_dist(int np, const double *coord, const int *ptype, int netot, 
 
 int fvm_gpu_tune(fvm_ctx *x, int variant, int vertices_per_block, int xcd_remap)
 {
+    if (!ctx_ok(x)) return FVM_E_INPUT;
+    if (int rc = settle(x)) return rc;
     return configure(x, variant, vertices_per_block, xcd_remap);
 }
 
 int fvm_gpu_set_cl(fvm_ctx *x, double cl)
 {
+    if (!ctx_ok(x)) return FVM_E_INPUT;
+    if (int rc = settle(x)) return rc;
     x->hS.cl = cl;
-    DevState h = x->hS;
-    HIPCHK(hipMemcpyAsync(&x->S->cl, &h.cl, sizeof(double), hipMemcpyHostToDevice, x->stream));
+    const double h = cl;
+    HIPCHK_S(x->stream, hipMemcpyAsync(&x->S->cl, &h, sizeof(double), hipMemcpyHostToDevice, x->stream));
     if (x->nff > 0)
         hipLaunchKernelGGL(k_farfield_state, dim3(nblk(x->nff)), dim3(kBlock), 0, x->stream, x->d, x->c, cl);
-    HIPCHK(hipGetLastError());
+    HIPCHK_S(x->stream, hipGetLastError());
     HIPCHK(hipStreamSynchronize(x->stream));
     return FVM_OK;
 }
 
 int fvm_gpu_get_state(fvm_ctx *x, double *prim, double *nut, double *qx, double *qy, double *mu)
 {
+    if (!ctx_ok(x)) return FVM_E_INPUT;
+    if (int rc = settle(x)) return rc;
     const size_t np = (size_t)x->np;
+    const DevState h = no_errors();        // lives until the stream drains below
     if (x->qxy_stale && (qx || qy)) {
         // the gradients of the last residual: those of the last stage's
         // input (buffer C), through the residual kernel, which writes them
-        DevState h{};
-        h.err_index = INT_MAX;
-        h.err1_index = INT_MAX;
-        HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
+        HIPCHK_S(x->stream, hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
         launch_residual(x, x->S_hook, x->pbuf[2], x->res_tmp);
-        HIPCHK(hipGetLastError());
+        HIPCHK_S(x->stream, hipGetLastError());
         x->qxy_stale = false;
     }
     std::vector<double> a(4 * np), b(4 * np), q(4 * np), r(np), m(np);
-    HIPCHK(hipMemcpyAsync(a.data(), x->d.prim, 32 * np, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(r.data(), x->d.nut, 8 * np, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(b.data(), x->d.qx, 32 * np, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(q.data(), x->d.qy, 32 * np, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(m.data(), x->d.mu, 8 * np, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(a.data(), x->d.prim, 32 * np, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(r.data(), x->d.nut, 8 * np, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(b.data(), x->d.qx, 32 * np, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(q.data(), x->d.qy, 32 * np, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(m.data(), x->d.mu, 8 * np, hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     if (prim) to_caller(x, a, 4, prim);
     if (nut) to_caller(x, r, 1, nut);
@@ -2827,23 +2911,23 @@ int fvm_gpu_get_state(fvm_ctx *x, double *prim, double *nut, double *qx, double 
 
 int fvm_gpu_residu(fvm_ctx *x, const double *prim, const double *nut, double *res)
 {
+    if (!ctx_ok(x) || !prim || !res) return FVM_E_INPUT;
+    if (int rc = settle(x)) return rc;
     const int np = x->np;
     // the solver's own state is put aside and restored afterwards
     HIPCHK(hipMemcpyAsync(x->prim_bak, x->d.prim, sizeof(double) * 4 * np, hipMemcpyDeviceToDevice, x->stream));
     HIPCHK(hipMemcpyAsync(x->nut_bak, x->d.nut, sizeof(double) * np, hipMemcpyDeviceToDevice, x->stream));
     std::vector<double> primv = to_device(x, prim, 4), nutv;
-    HIPCHK(hipMemcpyAsync(x->d.prim, primv.data(), sizeof(double) * 4 * np, hipMemcpyHostToDevice, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->d.prim, primv.data(), sizeof(double) * 4 * np, hipMemcpyHostToDevice, x->stream));
     if (nut && x->par.iflow == 3) {
         nutv = to_device(x, nut, 1);
-        HIPCHK(hipMemcpyAsync(x->d.nut, nutv.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
+        HIPCHK_S(x->stream, hipMemcpyAsync(x->d.nut, nutv.data(), sizeof(double) * np, hipMemcpyHostToDevice, x->stream));
     }
     // the hook keeps the iteration's error state apart, but shares cl for
     // the farfield vortex correction
-    DevState h{};
-    h.err_index = INT_MAX;
-    h.err1_index = INT_MAX;
+    DevState h = no_errors();
     h.cl = x->hS.cl;
-    HIPCHK(hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->S_hook, &h, sizeof h, hipMemcpyHostToDevice, x->stream));
     if (x->variant == 3) {
         launch_residual(x, x->S_hook, x->d.prim, x->res_tmp);
         x->qxy_stale = false;                   // the hook's gradients, as in the reference
@@ -2855,12 +2939,12 @@ int fvm_gpu_residu(fvm_ctx *x, const double *prim, const double *nut, double *re
         else
             hipLaunchKernelGGL(k_res<false>, dim3(nblk(np)), dim3(kBlock), 0, x->stream, x->d, x->res_tmp);
     }
-    HIPCHK(hipGetLastError());
+    HIPCHK_S(x->stream, hipGetLastError());
     std::vector<double> resv(4 * (size_t)np);
-    HIPCHK(hipMemcpyAsync(resv.data(), x->res_tmp, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(&h, x->S_hook, sizeof h, hipMemcpyDeviceToHost, x->stream));
-    HIPCHK(hipMemcpyAsync(x->d.prim, x->prim_bak, sizeof(double) * 4 * np, hipMemcpyDeviceToDevice, x->stream));
-    HIPCHK(hipMemcpyAsync(x->d.nut, x->nut_bak, sizeof(double) * np, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(resv.data(), x->res_tmp, sizeof(double) * 4 * np, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(&h, x->S_hook, sizeof h, hipMemcpyDeviceToHost, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->d.prim, x->prim_bak, sizeof(double) * 4 * np, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->d.nut, x->nut_bak, sizeof(double) * np, hipMemcpyDeviceToDevice, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     to_caller(x, resv, 4, res);
     if (h.err_code) return h.err_code;
@@ -2871,6 +2955,8 @@ int fvm_gpu_iterate(fvm_ctx *x, int n, double *hist, int *ndone, int *err_index)
 {
     if (ndone) *ndone = 0;
     if (err_index) *err_index = 0;
+    if (!ctx_ok(x) || n < 0) return FVM_E_INPUT;
+    if (x->batch) return FVM_E_INPUT;           // a member advances with its batch
     int done_total = 0;
     while (done_total < n && !x->hS.done) {
         int chunk = std::min(n - done_total, kHistCap);
@@ -2878,24 +2964,26 @@ int fvm_gpu_iterate(fvm_ctx *x, int n, double *hist, int *ndone, int *err_index)
         x->qxy_stale = x->variant == 3 && x->par.iflow == 1;
         if (x->graphs_ok && !x->graph_n) {
             if (build_graph(x, kGraphIters, &x->graph_n) || build_graph(x, 1, &x->graph_1)) {
+                drop_graphs(x);                 // none or both
                 x->graphs_ok = false;
-                (void)hipGetLastError();
             }
         }
         if (x->graphs_ok) {
             int q = chunk / kGraphIters, r = chunk % kGraphIters;
-            for (int i = 0; i < q; ++i) HIPCHK(hipGraphLaunch(x->graph_n, x->stream));
-            for (int i = 0; i < r; ++i) HIPCHK(hipGraphLaunch(x->graph_1, x->stream));
+            for (int i = 0; i < q; ++i) HIPCHK_S(x->stream, hipGraphLaunch(x->graph_n, x->stream));
+            for (int i = 0; i < r; ++i) HIPCHK_S(x->stream, hipGraphLaunch(x->graph_1, x->stream));
         } else {
             for (int i = 0; i < chunk; ++i) one_iteration(x, nullptr);
         }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(&x->hS, x->S, sizeof(DevState), hipMemcpyDeviceToHost, x->stream));
+        HIPCHK_S(x->stream, hipGetLastError());
+        HIPCHK_S(x->stream, hipMemcpyAsync(&x->hS, x->S, sizeof(DevState), hipMemcpyDeviceToHost, x->stream));
         HIPCHK(hipStreamSynchronize(x->stream));
-        int got = x->hS.nhist;
-        if (hist && got > 0)
-            HIPCHK(hipMemcpy(hist + 6 * (size_t)done_total, x->d.hist, sizeof(double) * 6 * got,
-                             hipMemcpyDeviceToHost));
+        const int got = std::min(x->hS.nhist, chunk);
+        if (hist && got > 0) {
+            HIPCHK_S(x->stream, hipMemcpyAsync(hist + 6 * (size_t)done_total, x->d.hist, sizeof(double) * 6 * got,
+                                               hipMemcpyDeviceToHost, x->stream));
+            HIPCHK(hipStreamSynchronize(x->stream));
+        }
         done_total += got;
         if (got < chunk) break;
     }
@@ -2909,6 +2997,7 @@ int fvm_gpu_iterate(fvm_ctx *x, int n, double *hist, int *ndone, int *err_index)
 
 int fvm_gpu_time_kernels(fvm_ctx *x, int n, double *ms, const char **names, int nk)
 {
+    if (!ctx_ok(x) || x->batch) return FVM_E_INPUT;
     Tally t;
     for (int i = 0; i < n && !x->hS.done; ++i) {
         hipLaunchKernelGGL(k_begin_chunk, dim3(1), dim3(1), 0, x->stream, x->S);
@@ -2968,7 +3057,7 @@ int fvm_gpu_stream_copy(long long bytes, int reps, double *gbs)
 int fvm_gpu_time_stage(fvm_ctx *x, int n, double *ms)
 {
     *ms = 0.0;
-    if (x->variant != 3 || n <= 0) return FVM_E_INPUT;
+    if (!ctx_ok(x) || x->batch || x->variant != 3 || n <= 0) return FVM_E_INPUT;
     if (x->hS.done) return x->hS.status;
     // stage 1 reads the iteration-start buffer and writes buffer B only, so
     // repeating it changes nothing the next iteration reads
@@ -2995,6 +3084,7 @@ int fvm_gpu_time_stage(fvm_ctx *x, int n, double *ms)
 
 namespace {
 
+// (the batch's stream has drained: callers wait or synchronise first)
 void batch_free(fvm_batch *b)
 {
     if (b->graph_n) (void)hipGraphExecDestroy(b->graph_n);
@@ -3052,10 +3142,21 @@ int batch_rebuild(fvm_batch *b)
     if (!rc) rc = batch_upload(&b->d_vwork, vwork);
     if (!rc && hipMalloc((void **)&b->d_states, std::max<size_t>(m.size(), 1) * sizeof(DevState)) != hipSuccess)
         rc = FVM_E_NOMEM;
+    if (!rc && b->hs_cap < (int)m.size()) {
+        if (b->hs) (void)hipHostFree(b->hs);
+        b->hs = nullptr;
+        b->hs_cap = 0;
+        const int cap = std::max<int>(16, (int)m.size() * 2);
+        if (hipHostMalloc((void **)&b->hs, sizeof(DevState) * cap, hipHostMallocDefault) != hipSuccess) {
+            b->hs = nullptr;
+            rc = FVM_E_NOMEM;
+        } else {
+            b->hs_cap = cap;
+        }
+    }
     if (rc) { batch_free(b); return rc; }
     b->nwork = (int)work.size();
     b->nvwork = (int)vwork.size();
-    b->hs.assign(m.size(), DevState{});
     b->graphs_ok = true;
     b->dirty = false;
     return FVM_OK;
@@ -3100,14 +3201,14 @@ void batch_iteration(fvm_batch *b, bool mon_prev = false, bool final = true)
 
 int batch_graph(fvm_batch *b, int iters, hipGraphExec_t *out)
 {
-    hipGraph_t g = nullptr;
-    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return FVM_E_HIP;
+    *out = nullptr;
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        return FVM_E_HIP;
+    }
     const bool merge = monitors_merge(b->mem[0]->par, b->any_vortex && b->max_nff > 0);
     for (int i = 0; i < iters; ++i) batch_iteration(b, merge && i > 0, !merge || i == iters - 1);
-    if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return FVM_E_HIP;
-    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    return e == hipSuccess ? FVM_OK : FVM_E_HIP;
+    return end_capture(b->stream, out);
 }
 
 bool same_kind(const fvm_params &a, const fvm_params &b)
@@ -3126,6 +3227,7 @@ int member_status(const DevState &h)
 
 int fvm_batch_create(fvm_batch **out)
 {
+    if (!out) return FVM_E_INPUT;
     *out = nullptr;
     fvm_batch *b = new fvm_batch();
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return FVM_E_HIP; }
@@ -3147,9 +3249,19 @@ int batch_wait(fvm_batch *b)
 
 }  // namespace
 
+// a batch member's own calls first wait for its batch's launched chunk
+int settle(fvm_ctx *x)
+{
+    return x->batch ? batch_wait(x->batch) : FVM_OK;
+}
+
+namespace {
+
+}  // namespace
+
 int fvm_batch_add(fvm_batch *b, fvm_ctx *x)
 {
-    if (!b || !x || x->batch || x->variant != 3) return FVM_E_INPUT;
+    if (!batch_ok(b) || !ctx_ok(x) || x->batch || x->variant != 3) return FVM_E_INPUT;
     if (!b->mem.empty() && !same_kind(b->mem[0]->par, x->par)) return FVM_E_INPUT;
     int rc = batch_wait(b);
     if (rc) return rc;
@@ -3162,7 +3274,7 @@ int fvm_batch_add(fvm_batch *b, fvm_ctx *x)
 
 int fvm_batch_remove(fvm_batch *b, fvm_ctx *x)
 {
-    if (!b || !x || x->batch != b) return FVM_E_INPUT;
+    if (!batch_ok(b) || !ctx_ok(x) || x->batch != b) return FVM_E_INPUT;
     int rc = batch_wait(b);
     if (rc) return rc;
     b->mem.erase(std::find(b->mem.begin(), b->mem.end(), x));
@@ -3171,10 +3283,11 @@ int fvm_batch_remove(fvm_batch *b, fvm_ctx *x)
     return FVM_OK;
 }
 
-int fvm_batch_size(const fvm_batch *b) { return b ? (int)b->mem.size() : 0; }
+int fvm_batch_size(const fvm_batch *b) { return batch_ok(b) ? (int)b->mem.size() : 0; }
 
 int fvm_batch_launch(fvm_batch *b, int n)
 {
+    if (!batch_ok(b)) return FVM_E_INPUT;
     const int nm = (int)b->mem.size();
     if (n < 0 || n > kHistCap || b->pending) return FVM_E_INPUT;
     if (nm == 0 || n == 0) return FVM_OK;
@@ -3189,26 +3302,29 @@ int fvm_batch_launch(fvm_batch *b, int n)
     hipLaunchKernelGGL(k_begin_batch, dim3(nblk(nm)), dim3(kBlock), 0, b->stream, b->d_mem, nm);
     if (b->graphs_ok && !b->graph_n) {
         if (batch_graph(b, kGraphIters, &b->graph_n) || batch_graph(b, 1, &b->graph_1)) {
+            if (b->graph_n) (void)hipGraphExecDestroy(b->graph_n);
+            if (b->graph_1) (void)hipGraphExecDestroy(b->graph_1);
+            b->graph_n = b->graph_1 = nullptr;          // none or both
             b->graphs_ok = false;
-            (void)hipGetLastError();
         }
     }
     if (b->graphs_ok) {
         const int q = n / kGraphIters, r = n % kGraphIters;
-        for (int i = 0; i < q; ++i) HIPCHK(hipGraphLaunch(b->graph_n, b->stream));
-        for (int i = 0; i < r; ++i) HIPCHK(hipGraphLaunch(b->graph_1, b->stream));
+        for (int i = 0; i < q; ++i) HIPCHK_S(b->stream, hipGraphLaunch(b->graph_n, b->stream));
+        for (int i = 0; i < r; ++i) HIPCHK_S(b->stream, hipGraphLaunch(b->graph_1, b->stream));
     } else {
         for (int i = 0; i < n; ++i) batch_iteration(b);
     }
     hipLaunchKernelGGL(k_gather_batch, dim3(nblk(nm)), dim3(kBlock), 0, b->stream, b->d_mem, nm, b->d_states);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(b->hs.data(), b->d_states, sizeof(DevState) * nm, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK_S(b->stream, hipGetLastError());
+    HIPCHK_S(b->stream, hipMemcpyAsync(b->hs, b->d_states, sizeof(DevState) * nm, hipMemcpyDeviceToHost, b->stream));
     b->pending = true;
     return FVM_OK;
 }
 
 int fvm_batch_wait(fvm_batch *b, int *status)
 {
+    if (!batch_ok(b)) return FVM_E_INPUT;
     int rc = batch_wait(b);
     if (rc) return rc;
     if (status)
@@ -3218,6 +3334,7 @@ int fvm_batch_wait(fvm_batch *b, int *status)
 
 int fvm_batch_iterate(fvm_batch *b, int n, int *status)
 {
+    if (!batch_ok(b)) return FVM_E_INPUT;
     if (b->mem.empty()) return n > 0 ? FVM_E_INPUT : FVM_OK;
     int rc = batch_wait(b);
     for (int left = n; !rc && left > 0; left -= kHistCap) {
@@ -3235,17 +3352,23 @@ int fvm_batch_iterate(fvm_batch *b, int n, int *status)
 int fvm_batch_destroy(fvm_batch *b)
 {
     if (!b) return FVM_OK;
+    if (!batch_ok(b)) return FVM_E_INPUT;
     (void)batch_wait(b);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (fvm_ctx *x : b->mem) x->batch = nullptr;
     b->mem.clear();
     batch_free(b);
+    if (b->hs) (void)hipHostFree(b->hs);
+    b->hs = nullptr;
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    b->tag = 0;
     delete b;
     return FVM_OK;
 }
 
 int fvm_gpu_monitor(const fvm_ctx *x, double *mon)
 {
+    if (!ctx_ok(x) || !mon) return FVM_E_INPUT;
     const DevState &h = x->hS;
     mon[0] = (double)h.iter;
     mon[1] = h.fres;

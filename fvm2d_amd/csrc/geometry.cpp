@@ -16,6 +16,8 @@
 #include "../../include/fvm2d_gpu.h"
 
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -165,7 +167,9 @@ extern "C" int fvm_geometric(int np, int nt, const double *coord, int *elem, con
         Pt a = P[V(t, 0) - 1], b = P[V(t, 1) - 1], c = P[V(t, 2) - 1];
         elarea[i] = 0.5 * ((b.x - a.x) * (c.y - a.y) - (c.x - a.x) * (b.y - a.y));
     });
-    int bad_tri = 0;
+    // the reference stops at the first triangle (ascending) with a negative
+    // sub-area (src/areas.f:109-151): the lowest one over all threads
+    std::atomic<int> bad_tri{INT_MAX};
     parallel_for(np, [&](long i) {
         int p = (int)i + 1;
         double m = 0.0, cv = 0.0;
@@ -179,14 +183,17 @@ extern "C" int fvm_geometric(int np, int nt, const double *coord, int *elem, con
                 double y[5] = {a.y, 0.5 * (a.y + nxt.y), tc[t - 1].y, 0.5 * (a.y + prv.y), a.y};
                 double area = 0.0;
                 for (int k = 0; k < 4; ++k) area = area + x[k] * y[k + 1] - x[k + 1] * y[k];
-                if (area < 0.0) bad_tri = t;
+                if (area < 0.0) {
+                    int cur = bad_tri.load(std::memory_order_relaxed);
+                    while (t < cur && !bad_tri.compare_exchange_weak(cur, t, std::memory_order_relaxed)) {}
+                }
                 cv = cv + 0.5 * area;
             }
         }
         mcarea[i] = m;
         cvarea[i] = (cell_type == 1) ? m : cv;
     });
-    if (bad_tri) { dims[11] = bad_tri; return FVM_E_AREA; }
+    if (bad_tri.load() != INT_MAX) { dims[11] = bad_tri.load(); return FVM_E_AREA; }
 
     // wall distance (wall_dist.f:17-94), brute force over the solid edges
     if (iflow == 3) {
