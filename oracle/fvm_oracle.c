@@ -702,7 +702,26 @@ static void hllc_flux(const orc_ctx *c, const double *ds, const double *pl_, con
     } else {
         flux[0] = rr * ur; flux[1] = pr + rr * sq(ur); flux[2] = rr * ur * vr; flux[3] = (Er + pr) * ur;
     }
+    /* hllc_rotate (no counterpart in the reference, which leaves the flux
+     * in the edge frame, hllc_flux.f:105-154): momentum back to x-y with the
+     * transpose of the rotation applied to the velocities above */
+    if (c->in.hllc_rotate) {
+        double fx = ct * flux[1] - st * flux[2], fy = st * flux[1] + ct * flux[2];
+        flux[1] = fx;
+        flux[2] = fy;
+    }
     for (int i = 0; i < 4; i++) flux[i] = flux[i] * length;
+}
+
+/* One numerical flux on given states (test hook): iflux 1 Roe, 2 KFVS,
+ * 3 HLLC (with the context's hllc_rotate).  Returns ORC_E_NEG_SOUND where
+ * roe_flux.f:60-65 stops. */
+int orc_flux(const orc_ctx *c, int iflux, const double *ds, const double *pl, const double *pr, double *flux)
+{
+    if (iflux == 1) return roe_flux(c, ds, pl, pr, flux);
+    if (iflux == 2) { kfvs_flux(c, ds, pl, pr, flux); return ORC_OK; }
+    if (iflux == 3) { hllc_flux(c, ds, pl, pr, flux); return ORC_OK; }
+    return ORC_E_INPUT;
 }
 
 /* ----------------------------------------------------------- solid_flux.f:2-20 */

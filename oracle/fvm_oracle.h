@@ -27,6 +27,8 @@ typedef struct {
     int sa_compat;       /* 1: keep sa_model.f:57-62 and wall_dist.f:46 as shipped */
     double mach_inf, aoa_deg, rey, cfl, minres, xref, yref;
     double nut_factor;   /* initial nu_tilde = nut_factor*mul/rho (initialize.f:71 uses 0.1) */
+    int hllc_rotate;     /* 0: HLLC flux left in the edge frame as shipped
+                            (hllc_flux.f:105-154); 1: momentum rotated to x-y */
 } orc_input;
 
 typedef struct orc_ctx orc_ctx;
@@ -50,6 +52,12 @@ int orc_set(orc_ctx *c, const char *name, const void *in);
 
 /* Scalars: cl, cd, fres, fres1, fresi, iresi, iter, and freestream values. */
 double orc_scalar(const orc_ctx *c, const char *name);
+
+/* One numerical flux on given states (a test hook; no reference
+ * counterpart): iflux 1 Roe (roe_flux.f), 2 KFVS (kfvs_flux.f), 3 HLLC
+ * (hllc_flux.f, rotated per hllc_rotate); ds(2), prim(4) each side. */
+int orc_flux(const orc_ctx *c, int iflux, const double *ds, const double *pl, const double *pr,
+             double *flux);
 
 /* One call of residu (residu.f:2-91) on the current prim/nut. */
 int orc_residu(orc_ctx *c);

@@ -20,7 +20,8 @@ class OrcInput(ctypes.Structure):
                 ("istart", "iflow", "iterlast", "maxiter", "saveinterval", "iflux",
                  "ilimit", "vortex", "cell_type", "sa_compat")] + \
                [(n, ctypes.c_double) for n in
-                ("mach_inf", "aoa_deg", "rey", "cfl", "minres", "xref", "yref", "nut_factor")]
+                ("mach_inf", "aoa_deg", "rey", "cfl", "minres", "xref", "yref", "nut_factor")] + \
+               [("hllc_rotate", ctypes.c_int)]
 
 
 def build() -> str:
@@ -57,6 +58,8 @@ def lib():
         L.orc_iterate.restype = ctypes.c_int
         L.orc_iterate.argtypes = [vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_int)]
         L.orc_error_index.argtypes = [vp]
+        L.orc_flux.restype = ctypes.c_int
+        L.orc_flux.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -68,7 +71,7 @@ DIMS = ("np", "nt", "ne", "netot", "nsp", "nfp", "nbe", "nswe1", "nswe2", "nffe1
 def make_input(**kw) -> OrcInput:
     d = dict(istart=1, iflow=1, iterlast=0, maxiter=1000, saveinterval=10**9, iflux=1,
              ilimit=1, vortex=0, cell_type=1, sa_compat=1, mach_inf=0.5, aoa_deg=2.0,
-             rey=0.0, cfl=1.0, minres=0.0, xref=0.25, yref=0.0, nut_factor=0.1)
+             rey=0.0, cfl=1.0, minres=0.0, xref=0.25, yref=0.0, nut_factor=0.1, hllc_rotate=0)
     d.update(kw)
     return OrcInput(**d)
 
@@ -136,3 +139,14 @@ class Oracle:
 
     def error_index(self):
         return self.L.orc_error_index(self.h)
+
+    def flux(self, iflux, ds, pl, pr):
+        """One numerical flux (orc_flux) on primitive states pl, pr across ds."""
+        ds = np.ascontiguousarray(ds, dtype=np.float64)
+        pl = np.ascontiguousarray(pl, dtype=np.float64)
+        pr = np.ascontiguousarray(pr, dtype=np.float64)
+        f = np.zeros(4)
+        rc = self.L.orc_flux(self.h, iflux, ds.ctypes.data, pl.ctypes.data, pr.ctypes.data, f.ctypes.data)
+        if rc != 0:
+            raise RuntimeError(f"flux failed: {rc}")
+        return f
