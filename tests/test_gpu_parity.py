@@ -199,15 +199,15 @@ def test_convergence_stop_and_no_work_after():
 
 
 def test_workload_grid_513x129_matches_oracle():
-    """The 66k-vertex transonic workload grid: 12 iterations, bit-identical
+    """The 66k-vertex transonic workload grid: 200 iterations, bit-identical
     states, against the oracle on the same inputs."""
     from oracle.oracle import Oracle
     g = mesh.naca0012_513x129()
     kw = dict(iflow=1, iflux=1, ilimit=1, mach_inf=0.8, aoa_deg=1.25, cfl=1.0, maxiter=2000)
     o = Oracle(g.coord, g.elem, g.ptype, **kw)
-    ho, _ = o.iterate(12)
+    ho, _ = o.iterate(200)
     with Solver(g, Inputs.from_kwargs(**kw)) as s:
-        hg, _ = s.iterate(12)
+        hg, _ = s.iterate(200)
         st = s.get_state()
     np.testing.assert_array_equal(st["prim"], o.get("prim"))
     np.testing.assert_array_equal(hg[:, [0, 2, 3]], ho[:, [0, 2, 3]])
