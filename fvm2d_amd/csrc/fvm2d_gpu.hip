@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <type_traits>
 #include <vector>
@@ -1111,6 +1112,28 @@ k_patch(Dev d, Consts c, DevState *S, PatchArgs pa, int irk, const double4 *__re
 // staged through LDS in tiles; fmin over the same candidates gives the same
 // value in any order.  COMPAT keeps the reference's dy3 = x2 - x1
 // (src/wall_dist.f:46).
+// distance of x to the n solid edges a[k]-b[k] staged in LDS, folded into w
+template <bool COMPAT>
+__device__ __forceinline__ double wall_candidates(const double2 x, const double2 *ea, const double2 *eb, int n,
+                                                  double w)
+{
+    for (int k = 0; k < n; ++k) {
+        const double2 a = ea[k], b = eb[k];
+        const double dx1 = a.x - x.x, dy1 = a.y - x.y;
+        w = fmin(w, sqrt(dx1 * dx1 + dy1 * dy1));
+        const double dx2 = b.x - x.x, dy2 = b.y - x.y;
+        w = fmin(w, sqrt(dx2 * dx2 + dy2 * dy2));
+        const double dx3 = b.x - a.x;
+        const double dy3 = COMPAT ? b.x - a.x : b.y - a.y;
+        const double xi = -(dx1 * dx3 + dy1 * dy3) / (dx3 * dx3 + dy3 * dy3);
+        if (xi >= 0.0 && xi <= 1.0) {
+            const double xp = a.x + xi * dx3, yp = a.y + xi * dy3;
+            w = fmin(w, sqrt((x.x - xp) * (x.x - xp) + (x.y - yp) * (x.y - yp)));
+        }
+    }
+    return w;
+}
+
 template <bool COMPAT>
 __global__ void __launch_bounds__(kBlock) k_wall_dist(int np, const double2 *__restrict__ xy,
                                                      const int2 *__restrict__ sedge, int nse,
@@ -1125,24 +1148,61 @@ __global__ void __launch_bounds__(kBlock) k_wall_dist(int np, const double2 *__r
         const int j = base + threadIdx.x;
         if (j < nse) { int2 e = sedge[j]; ea[threadIdx.x] = xy[e.x]; eb[threadIdx.x] = xy[e.y]; }
         __syncthreads();
-        const int n = min(kBlock, nse - base);
-        for (int k = 0; k < n; ++k) {
-            const double2 a = ea[k], b = eb[k];
-            const double dx1 = a.x - x.x, dy1 = a.y - x.y;
-            w = fmin(w, sqrt(dx1 * dx1 + dy1 * dy1));
-            const double dx2 = b.x - x.x, dy2 = b.y - x.y;
-            w = fmin(w, sqrt(dx2 * dx2 + dy2 * dy2));
-            const double dx3 = b.x - a.x;
-            const double dy3 = COMPAT ? b.x - a.x : b.y - a.y;
-            const double xi = -(dx1 * dx3 + dy1 * dy3) / (dx3 * dx3 + dy3 * dy3);
-            if (xi >= 0.0 && xi <= 1.0) {
-                const double xp = a.x + xi * dx3, yp = a.y + xi * dy3;
-                w = fmin(w, sqrt((x.x - xp) * (x.x - xp) + (x.y - yp) * (x.y - yp)));
-            }
-        }
+        w = wall_candidates<COMPAT>(x, ea, eb, min(kBlock, nse - base), w);
         __syncthreads();
     }
     if (i < np) wd[i] = solid[i] ? 0.0 : w;
+}
+
+// ---- wall distance without allocation (fvm_gpu_wall_dist_noalloc) ----
+// The solid-edge list lives in a static device array: words 0-3 are a
+// counter, an overflow flag and a bad-edge flag, then up to kScratchEdges
+// int2 end-point pairs.  Every call on a device shares its copy.
+constexpr int kScratchEdges = 1 << 16;
+constexpr int kScratchInts = 4 + 2 * kScratchEdges;
+static __device__ __align__(256) int scratch[kScratchInts];
+
+__global__ void k_scratch_clear(int *sc)
+{
+    if (threadIdx.x < 4) sc[threadIdx.x] = 0;
+}
+
+// the edges with both ends solid (ptype 3), in any order (the distance is a
+// minimum over them, which the order does not change)
+__global__ void __launch_bounds__(kBlock) k_collect_solid_edges(int np, int netot, const int *__restrict__ edge,
+                                                               const int *__restrict__ ptype, int *sc)
+{
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= netot) return;
+    const int a = edge[2 * e] - 1, b = edge[2 * e + 1] - 1;
+    if (a < 0 || a >= np || b < 0 || b >= np) { atomicOr(&sc[2], 1); return; }
+    if (ptype[a] != 3 || ptype[b] != 3) return;
+    const int k = atomicAdd(&sc[0], 1);
+    if (k < kScratchEdges) reinterpret_cast<int2 *>(sc + 4)[k] = make_int2(a, b);
+    else atomicOr(&sc[1], 1);
+}
+
+// k_wall_dist over the collected list; solid points get 1 directly (the
+// value src/wall_dist.f:90-92 leaves there)
+template <bool COMPAT>
+__global__ void __launch_bounds__(kBlock) k_wall_dist_list(int np, const double2 *__restrict__ xy,
+                                                          const int *__restrict__ ptype, const int *sc,
+                                                          double *__restrict__ wd)
+{
+    __shared__ double2 ea[kBlock], eb[kBlock];
+    const int nse = (sc[1] | sc[2]) ? 0 : min(sc[0], kScratchEdges);
+    const int2 *sedge = reinterpret_cast<const int2 *>(sc + 4);
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const double2 x = i < np ? xy[i] : make_double2(0.0, 0.0);
+    double w = 1.0e10;
+    for (int base = 0; base < nse; base += kBlock) {
+        const int j = base + threadIdx.x;
+        if (j < nse) { int2 e = sedge[j]; ea[threadIdx.x] = xy[e.x]; eb[threadIdx.x] = xy[e.y]; }
+        __syncthreads();
+        w = wall_candidates<COMPAT>(x, ea, eb, min(kBlock, nse - base), w);
+        __syncthreads();
+    }
+    if (i < np) wd[i] = ptype[i] == 3 ? 1.0 : w;
 }
 
 // Start of an iterate call: history rows from zero, and no first-stage error
@@ -2856,6 +2916,56 @@ int fvm_gpu_wall_dist(int np, const double *coord, const int *ptype, int netot, 
     }
     for (int p = 0; p < np; ++p)
         if (solid[p]) wd[p] = 1.0;
+    return FVM_OK;
+}
+
+// Device address of `scratch` on each device, looked up once per device
+// with that device current (the array's name is not a device address in
+// host code).
+static int *scratch_ptr(int dev)
+{
+    static std::mutex mu;
+    static int *ptr[64] = {nullptr};
+    if (dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        ptr[dev] = static_cast<int *>(p);
+    }
+    return ptr[dev];
+}
+
+int fvm_gpu_wall_dist_noalloc(int np, const double *coord, const int *ptype, int netot, const int *edge,
+                              int sa_compat, double *wd, void *stream)
+{
+    if (np <= 0 || netot <= 0 || !coord || !ptype || !edge || !wd) return FVM_E_INPUT;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    int *sc = scratch_ptr(dev);
+    if (!sc) return FVM_E_HIP;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_scratch_clear, dim3(1), dim3(64), 0, st, sc);
+    hipLaunchKernelGGL(k_collect_solid_edges, dim3(nblk(netot)), dim3(kBlock), 0, st, np, netot, edge, ptype, sc);
+    if (sa_compat)
+        hipLaunchKernelGGL(k_wall_dist_list<true>, dim3(nblk(np)), dim3(kBlock), 0, st, np,
+                           reinterpret_cast<const double2 *>(coord), ptype, sc, wd);
+    else
+        hipLaunchKernelGGL(k_wall_dist_list<false>, dim3(nblk(np)), dim3(kBlock), 0, st, np,
+                           reinterpret_cast<const double2 *>(coord), ptype, sc, wd);
+    HIPCHK_S(st, hipGetLastError());
+    int flags[4] = {0, 0, 0, 0};
+    HIPCHK_S(st, hipMemcpyAsync(flags, sc, sizeof flags, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (flags[2]) return FVM_E_INPUT;
+    if (flags[1]) {
+        std::fprintf(stderr, "fvm_gpu_wall_dist_noalloc: %d solid edges exceed the static scratch of %d\n",
+                     flags[0], kScratchEdges);
+        return FVM_E_SIZE;
+    }
     return FVM_OK;
 }
 

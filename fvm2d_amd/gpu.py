@@ -72,6 +72,8 @@ def lib():
     L.fvm_gpu_set_state.argtypes = [vp, vp, vp]
     L.fvm_gpu_wall_dist.restype = ci
     L.fvm_gpu_wall_dist.argtypes = [ci, vp, vp, ci, vp, ci, vp, vp]
+    L.fvm_gpu_wall_dist_noalloc.restype = ci
+    L.fvm_gpu_wall_dist_noalloc.argtypes = [ci, vp, vp, ci, vp, ci, vp, vp]
     L.fvm_gpu_tune.restype = ci
     L.fvm_gpu_tune.argtypes = [vp, ci, ci, ci]
     L.fvm_gpu_set_cl.restype = ci

@@ -104,6 +104,20 @@ int fvm_geometric(int np, int nt, const double *coord, int *elem, const int *pty
 int fvm_gpu_wall_dist(int np, const double *coord, const int *ptype, int netot, const int *edge,
                       int sa_compat, double *wd, double *wd1);
 
+/* fvm_gpu_wall_dist for callers that cannot allocate (code between graph
+ * captures, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It allocates nothing: coord(2,np),
+ * ptype(np), edge(2,netot) (1-based) and wd(np) are DEVICE pointers of the
+ * caller, `stream` a hipStream_t (NULL: the null stream).  Its list of solid
+ * edges goes to a static device array of at most 65536 edges, which each GPU
+ * holds until the process ends; more gives FVM_E_SIZE, an edge end outside
+ * 1..np FVM_E_INPUT.  wd is bit for bit fvm_gpu_wall_dist's (wd1 is not
+ * computed).  The call waits for its stream.  All calls on one GPU share the
+ * array: calls that overlap on one GPU (several host threads or streams)
+ * race and must be serialised by the caller. */
+int fvm_gpu_wall_dist_noalloc(int np, const double *coord, const int *ptype, int netot, const int *edge,
+                              int sa_compat, double *wd, void *stream);
+
 /* Uploads the mesh, builds the device layout and sets the free-stream state
  * (src/initialize.f:14-90 with Start 1).  One context = one solve, on the
  * current HIP device, with its own stream. */
