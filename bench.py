@@ -11,8 +11,10 @@ JSON line with the throughput in vertex-iterations/s, the roofline of the
 dominant kernel (algorithmic bytes per launch / its HIP-event duration on
 the solver's stream) and the CPU baseline (the oracle, single core, on a
 bounded sample of the same workload); beside them split precision, batches
-of solves, and the serving loop's p50/p99 solve latency on a bounded stream
-of requests.
+of solves, the serving loop's p50/p99 latency on a bounded stream of
+requests solved to a 1e-6 residual drop (with converged / capped counts), and
+the viscous workloads (laminar 1025x257; SA-RANS 2049x513, a batch of 8) in
+FP64 and split precision with their own rooflines.
 """
 from __future__ import annotations
 
@@ -40,6 +42,24 @@ def algorithmic_bytes(kernel: str, npt: int, nt: int, ne: int) -> float | None:
     G, F, U = 12 * nt + 120 * npt, 24 * ne + 144 * npt, 144 * npt
     return {"k_grad": G, "k_grad2": G, "k_flux": F, "k_update": U, "k_fu2": F + U,
             "k_patch": G + F + U, "k_tstep": 112 * npt}.get(kernel)
+
+
+def stage_bytes(flow: int, npt: int, nt: int, ne: int) -> float:
+    """Algorithmic bytes of one RK stage (gradient + flux + update passes):
+    Euler 12 nt + 24 ne + 408 np; viscous flow adds the Sutherland / mu pass
+    (48 np), the Galerkin viscous terms in the gradient pass (8 nt + 56 np)
+    and the residual read-modify-write in the flux pass (32 np)."""
+    b = 12 * nt + 24 * ne + 408 * npt
+    return b + (8 * nt + 136 * npt if flow != 1 else 0)
+
+
+def iteration_bytes(flow: int, npt: int, nt: int, ne: int) -> float:
+    """Algorithmic bytes of one iteration: three stages, save_old + time_step
+    + conres (176 np; the viscous time step also reads mu, 8 np) and, for SA,
+    the SA step (its triangle, edge and vertex passes: 20 nt + 24 ne +
+    304 np)."""
+    b = 3 * stage_bytes(flow, npt, nt, ne) + 176 * npt + (8 * npt if flow != 1 else 0)
+    return b + (20 * nt + 24 * ne + 304 * npt if flow == 3 else 0)
 
 
 def traffic_from_profiles(kernel: str):
@@ -215,61 +235,71 @@ def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
     return out
 
 
-def serving_leg(n: int = 32, seed: int = 11, max_iters: int = 300) -> dict:
+def serving_leg(n: int = 32, seed: int = 11, max_iters: int = 20000) -> dict:
     """Solve latency of the serving loop (fvm2d_amd/serving.py) on a bounded
-    stream: n seeded requests of 600-6000 vertices (NACA 4-digit airfoils,
-    70/30 Euler/SA-RANS), each to a 1e-6 residual drop or max_iters
-    iterations, Euler in split precision and SA-RANS in FP64.  Offered at 400 requests/s (faster than
-    the loop serves them), the run gives the saturation rate in solves/s; then the same requests arrive as Poisson
-    processes at 50 % and 90 % of that rate.  A latency runs from a request's
-    arrival to its result: host meshing and preprocessing, admission, the
-    solve."""
+    stream: n seeded requests of 1000-4000 vertices (NACA 4-digit airfoils,
+    70/30 Euler/SA-RANS), each solved to a 1e-6 residual drop, with an
+    iteration cap of max_iters (Euler in split precision, SA-RANS in FP64).
+    Offered all at once, the run gives the saturation rate in solves/s; then
+    the same requests arrive as Poisson processes at 50 % and 90 % of that
+    rate.  A latency runs from a request's arrival to its result: host
+    meshing and preprocessing, admission, the solve.  Latencies are given over
+    all requests and over the converged ones, with the counts."""
+    import numpy as np
     from fvm2d_amd import serving
-    cfg = serving.ServeConfig(max_iters=max_iters, step_vertex_iters=2e5, minres=1e-6,
-                              size_classes=(2500,))
+    cfg = serving.ServeConfig(max_iters=max_iters, minres=1e-6, size_classes=(2500,))
 
     def run(rate: float) -> dict:
-        reqs = serving.poisson_arrivals(serving.workload(n, seed, np_min=600, np_max=6000),
+        reqs = serving.poisson_arrivals(serving.workload(n, seed, np_min=1000, np_max=4000),
                                         rate, seed)
         srv = serving.Server(cfg)
-        res = srv.run(reqs, timeout=120.0)
+        res = srv.run(reqs, timeout=240.0)
         s = serving.summary(res, srv.wall)
-        out = {"offered_per_s": round(rate, 2), "solves": s["solves"], "wall_s": s["wall_s"],
-               "solves_per_s": round(s["solves_per_s"], 2),
+        conv = [r.latency for r in res if r.status == 1]
+        out = {"offered_per_s": round(rate, 3), "solves": s["solves"], "wall_s": s["wall_s"],
+               "solves_per_s": round(s["solves_per_s"], 3),
                "vertex_iters_per_s": round(s["vertex_iters_per_s"], 1),
                "p50_latency_s": round(s["p50_latency_s"], 4),
                "p99_latency_s": round(s["p99_latency_s"], 4),
                "converged": s["converged"], "capped": s["capped"], "failed": s["failed"],
+               "p50_converged_latency_s": round(float(np.percentile(conv, 50)), 4) if conv else None,
+               "p99_converged_latency_s": round(float(np.percentile(conv, 99)), 4) if conv else None,
+               "mean_iterations": round(float(np.mean([r.iters for r in res])), 1),
                "peak_in_flight": srv.peak_in_flight}
         if srv.errors:
             out["errors"] = srv.errors
         return out
 
-    run(400.0)                          # first pass: one-time costs of the loop
-    sat = run(400.0)
+    sat = run(0.0)                      # every request at t = 0: the saturation rate
     rate = sat["solves_per_s"]
-    return {"workload": f"{n} requests, NACA 4-digit O-grids of 600-6000 vertices, 70/30 "
-                        f"Euler/SA-RANS (Euler split precision, SA-RANS FP64), residual drop 1e-6 or "
-                        f"{max_iters} iterations",
+    return {"workload": f"{n} requests, NACA 4-digit O-grids of 1000-4000 vertices, 70/30 "
+                        f"Euler/SA-RANS (Euler split precision, SA-RANS FP64), each to a 1e-6 "
+                        f"residual drop, iteration cap {max_iters}",
             "saturation": sat, "load_50": run(0.5 * rate), "load_90": run(0.9 * rate)}
 
 
 def config_legs(iters_lam: int, iters_rans: int) -> dict:
-    """Split precision against FP64 on the viscous workloads.
+    """The viscous workloads, FP64 and split precision.
 
     Laminar: NACA0012 1025x257 O-grid (wall spacing 1e-4, ratio 1.1), M 0.5,
     alpha 0, Re 5000, Roe, limited.  RANS: NACA0012 2049x513 O-grid (wall
-    spacing 4e-6, ratio 1.15), M 0.15, Re 6e6, SA with freestream nu_tilde =
-    3 nu, a batch of 8 solves at alpha = 0, 2, ..., 14 deg in one launch per
-    RK stage.  Both run a bounded number of iterations from the free stream
-    (stated), FP64 and split; the differences are taken at the same
-    iteration: CL, CD and the L2 density-residual history."""
+    spacing 4e-6, ratio 1.15), M 0.15, Re 6e6, SA in its corrected form
+    (sa_compat 0: nu_tilde gradient at each triangle's corners, exact wall
+    distance) with freestream nu_tilde = 3 nu, a batch of 8 solves at alpha =
+    0, 2, ..., 14 deg in one launch per RK stage.  Both run a bounded number
+    of iterations from the free stream (stated); the split-minus-FP64
+    differences of CL, CD and of the L2 density-residual history are taken at
+    the same iteration.  Rooflines: the laminar stage kernel over 100
+    back-to-back launches (HIP events on the solver's stream) against the
+    viscous stage bytes; the RANS batch over whole iterations (its kernels and
+    the gaps between them) against the SA iteration bytes of its 8 solves."""
     import numpy as np
     from fvm2d_amd import mesh
     from fvm2d_amd.solver import Batch, Inputs, Solver, geometric
     out = {}
     g = mesh.naca0012_1025x257()
     geo = geometric(g.coord, g.elem, g.ptype, iflow=2)
+    d = geo.dims
     kw = dict(iflow=2, iflux=1, ilimit=1, mach_inf=0.5, aoa_deg=0.0, rey=5000.0, cfl=1.0,
               maxiter=10**9, minres=0.0)
     lam = {}
@@ -279,20 +309,31 @@ def config_legs(iters_lam: int, iters_rans: int) -> dict:
             x.iterate(2)
             t0 = time.perf_counter()
             h, _ = x.iterate(iters_lam)
-            lam[prec] = (h, np.float64(geo.dims["np"]) * iters_lam / (time.perf_counter() - t0))
+            rate = d["np"] * iters_lam / (time.perf_counter() - t0)
+            stage_ms = x.time_stage(100)
+            lam[prec] = (h, rate, stage_ms)
     h0, h1 = lam[0][0], lam[1][0]
+    B = stage_bytes(2, d["np"], d["nt"], d["ne"])
     out["laminar_1025x257"] = {
-        "iterations": iters_lam, "np": geo.dims["np"],
+        "iterations": iters_lam, "np": d["np"],
         "fp64_vertex_iters_per_s": round(float(lam[0][1]), 1),
         "split_vertex_iters_per_s": round(float(lam[1][1]), 1),
-        "cl_fp64": float(h0[-1, 4]), "cd_fp64": float(h0[-1, 5]),
+        "roofline": {"bound": "hbm", "kernel": "stage kernel (laminar, FP64)",
+                     "achieved": round(B / (lam[0][2] * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(B / (lam[0][2] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes_per_launch": B, "kernel_ms_per_launch": round(lam[0][2], 5),
+                     "split_kernel_ms_per_launch": round(lam[1][2], 5)},
+        "fres_drop_fp64": float(h0[-1, 1]), "cl_fp64": float(h0[-1, 4]), "cd_fp64": float(h0[-1, 5]),
         "dcl": abs(float(h1[-1, 4] - h0[-1, 4])), "dcd": abs(float(h1[-1, 5] - h0[-1, 5])),
         "max_rel_dfres": float(np.max(np.abs(h1[:, 1] - h0[:, 1]) / np.abs(h0[:, 1])))}
     del geo
     g = mesh.naca0012_2049x513()
-    geo = geometric(g.coord, g.elem, g.ptype, iflow=3, wall_distance="gpu")
+    t0 = time.perf_counter()
+    geo = geometric(g.coord, g.elem, g.ptype, iflow=3, sa_compat=0, wall_distance="gpu")
+    prep_s = time.perf_counter() - t0
+    d = geo.dims
     kw = dict(iflow=3, iflux=1, ilimit=1, mach_inf=0.15, rey=6.0e6, cfl=1.0, maxiter=10**9,
-              minres=0.0, nut_factor=3.0)
+              minres=0.0, nut_factor=3.0, sa_compat=0)
     alphas = [2.0 * k for k in range(8)]
     rans = {}
     for prec in (0, 1):
@@ -309,12 +350,20 @@ def config_legs(iters_lam: int, iters_rans: int) -> dict:
             mons = [x.monitor()[0] for x in sol]
         for x in sol:
             x.close()
-        rans[prec] = (mons, len(sol) * geo.dims["np"] * iters_rans / dt, st)
+        rans[prec] = (mons, len(sol) * d["np"] * iters_rans / dt, st, dt)
+    Bi = len(alphas) * iteration_bytes(3, d["np"], d["nt"], d["ne"])
+    it_ms = rans[0][3] / iters_rans * 1e3
     out["rans_2049x513_batch8"] = {
-        "iterations": iters_rans, "np": geo.dims["np"], "alphas_deg": alphas,
+        "iterations": iters_rans, "np": d["np"], "alphas_deg": alphas, "sa": "corrected (sa_compat 0)",
         "fp64_vertex_iters_per_s": round(rans[0][1], 1),
         "split_vertex_iters_per_s": round(rans[1][1], 1),
+        "roofline": {"bound": "hbm", "kernel": "whole iteration of the batch (FP64)",
+                     "achieved": round(Bi / (it_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(Bi / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes_per_iteration": Bi, "ms_per_iteration": round(it_ms, 4)},
+        "preprocessing_s": round(prep_s, 2),
         "status_fp64": rans[0][2], "status_split": rans[1][2],
+        "fres_fp64": [m["fres"] for m in rans[0][0]],
         "cl_fp64": [m["cl"] for m in rans[0][0]], "cd_fp64": [m["cd"] for m in rans[0][0]],
         "dcl": [abs(a["cl"] - b["cl"]) for a, b in zip(rans[0][0], rans[1][0])],
         "dcd": [abs(a["cd"] - b["cd"]) for a, b in zip(rans[0][0], rans[1][0])]}
@@ -330,9 +379,9 @@ def rans_converge(max_iters: int, chunk: int = 1000) -> dict:
     from fvm2d_amd import mesh
     from fvm2d_amd.solver import Batch, Inputs, Solver, geometric
     g = mesh.naca0012_2049x513()
-    geo = geometric(g.coord, g.elem, g.ptype, iflow=3, wall_distance="gpu")
+    geo = geometric(g.coord, g.elem, g.ptype, iflow=3, sa_compat=0, wall_distance="gpu")
     kw = dict(iflow=3, iflux=1, ilimit=1, mach_inf=0.15, rey=6.0e6, cfl=1.0, maxiter=max_iters,
-              minres=1e-8, nut_factor=3.0)
+              minres=1e-8, nut_factor=3.0, sa_compat=0)
     alphas = [2.0 * k for k in range(8)]
     res = {}
     for prec in (0, 1):
@@ -412,6 +461,12 @@ def main() -> int:
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the split-precision and batched legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the laminar 1025x257 and RANS 2049x513 batch-of-8 legs")
+    ap.add_argument("--lam-iters", type=int, default=5000,
+                    help="iterations of the laminar 1025x257 leg (FP64 and split)")
+    ap.add_argument("--rans-iters", type=int, default=500,
+                    help="iterations of the RANS 2049x513 batch-of-8 leg (FP64 and split)")
     ap.add_argument("--variant", type=int, default=None, help="kernel layout 1 or 2 (default: tuned)")
     ap.add_argument("--vb", type=int, default=64, help="vertices per workgroup (variant 2)")
     ap.add_argument("--tune", action="store_true",
@@ -550,6 +605,7 @@ def main() -> int:
                          "dcl, dcd against this FP64 run"}
     batched = None if a.no_batch else batch_legs(g, s.geo, kw, value, a.warmup)
     served = None if a.no_serving else serving_leg()
+    viscous = None if a.no_configs else config_legs(a.lam_iters, a.rans_iters)
 
     line = {
         "metric": "vertex-iters/s & %HBM roofline vs plain FP64; p50/p99 solve latency",
@@ -562,14 +618,20 @@ def main() -> int:
                                % {1: "Roe", 2: "KFVS", 3: "HLLC"}[a.flux],
                    "np": npt, "nt": nt, "ne": ne, "precision": "fp64", "launch": launch},
         "roofline": roofline, "cpu_baseline": cpu, "split_precision": split, "batched": batched,
-        "serving": served,
+        "serving": served, "viscous_configs": viscous,
         "cl": float(hist[-1, 4]), "cd": float(hist[-1, 5]),
     }
     if stopped:
         line["stopped"] = stopped
     if served:                          # the latency part of the metric, at 90% load
-        line["p50_latency_s"] = served["load_90"]["p50_latency_s"]
-        line["p99_latency_s"] = served["load_90"]["p99_latency_s"]
+        l90 = served["load_90"]
+        line["p50_latency_s"] = l90["p50_latency_s"]
+        line["p99_latency_s"] = l90["p99_latency_s"]
+        line["latency_of"] = (f"serving at 90% of saturation, arrival to result of {l90['solves']} "
+                              f"requests of 1000-4000 vertices solved to a 1e-6 residual drop: "
+                              f"{l90['converged']} converged, {l90['capped']} capped at "
+                              f"{served['workload'].rsplit(' ', 1)[-1]} iterations, {l90['failed']} "
+                              f"stopped on a reference STOP")
     if cpu:
         line["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)

@@ -29,3 +29,17 @@ def test_more_than_one_gpu_is_refused():
 def test_traffic_summary_is_committed():
     t = bench.traffic_from_profiles("k_patch")
     assert isinstance(t, int) and 0 < t < 2 * 33251328
+
+
+def test_viscous_and_sa_byte_models():
+    # per stage, viscous flow adds 8 nt + 136 np; per iteration SA adds
+    # 20 nt + 24 ne + 304 np (the SA step's passes)
+    npt, nt, ne = 1050624, 2097152, 3147776
+    e = 12 * nt + 24 * ne + 408 * npt
+    assert bench.stage_bytes(1, npt, nt, ne) == e
+    assert bench.stage_bytes(2, npt, nt, ne) == e + 8 * nt + 136 * npt
+    lam = 3 * (e + 8 * nt + 136 * npt) + 176 * npt + 8 * npt
+    assert bench.iteration_bytes(2, npt, nt, ne) == lam
+    assert bench.iteration_bytes(3, npt, nt, ne) == lam + 20 * nt + 24 * ne + 304 * npt
+    # about 2.57 kB per vertex-iteration for SA on this grid
+    assert 2560 < bench.iteration_bytes(3, npt, nt, ne) / npt < 2580
