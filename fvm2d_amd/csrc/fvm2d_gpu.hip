@@ -30,6 +30,7 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -2216,19 +2217,20 @@ struct PatchHost {
 // static LDS of the block reduction (5 KiB) and a margin
 constexpr int kLdsDefault = 58 * 1024;
 
-int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<int> &aptr,
-                  const std::vector<int> &adj, const std::vector<int> &v2t_ptr,
-                  const std::vector<int> &v2t, const std::vector<int4> &tri,
-                  const std::vector<int> &v2e_ptr, const std::vector<int> &v2e,
-                  const std::vector<int2> &enode, const std::vector<int> &v2b_ptr,
-                  const std::vector<int> &v2b, const std::vector<int2> &bnode, int iflow,
-                  bool split, int lds_limit, PatchHost &ph)
+// Tables of patches [k0, k1) into ph (offsets in the header relative to ph's
+// arrays)
+int build_patch_range(int np, const std::vector<int> &patch_ptr, const std::vector<int> &aptr,
+                      const std::vector<int> &adj, const std::vector<int> &v2t_ptr,
+                      const std::vector<int> &v2t, const std::vector<int4> &tri,
+                      const std::vector<int> &v2e_ptr, const std::vector<int> &v2e,
+                      const std::vector<int2> &enode, const std::vector<int> &v2b_ptr,
+                      const std::vector<int> &v2b, const std::vector<int2> &bnode, int iflow,
+                      bool split, int k0, int k1, PatchHost &ph)
 {
     const int ntri = (int)tri.size(), nedge = (int)enode.size(), nbnd = (int)bnode.size();
     std::vector<int> loc(np, -1), tloc(ntri, -1), eloc(nedge, -1), bloc(nbnd, -1);
     std::vector<int> lv, tl, el, bl;
-    const int npatch = (int)patch_ptr.size() - 1;
-    for (int k = 0; k < npatch; ++k) {
+    for (int k = k0; k < k1; ++k) {
         const int v0 = patch_ptr[k], v1 = patch_ptr[k + 1], NP = v1 - v0;
         if (NP > kBlock) return FVM_E_SIZE;
         lv.clear(); tl.clear(); el.clear(); bl.clear();
@@ -2328,6 +2330,56 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
         for (int t : tl) tloc[t] = -1;
         for (int e : el) eloc[e] = -1;
         for (int b : bl) bloc[b] = -1;
+    }
+    return FVM_OK;
+}
+
+// Tables of every patch; large meshes build ranges of patches on host
+// threads and join them in patch order (the same tables as one thread).
+int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<int> &aptr,
+                  const std::vector<int> &adj, const std::vector<int> &v2t_ptr,
+                  const std::vector<int> &v2t, const std::vector<int4> &tri,
+                  const std::vector<int> &v2e_ptr, const std::vector<int> &v2e,
+                  const std::vector<int2> &enode, const std::vector<int> &v2b_ptr,
+                  const std::vector<int> &v2b, const std::vector<int2> &bnode, int iflow,
+                  bool split, int lds_limit, PatchHost &ph)
+{
+    const int npatch = (int)patch_ptr.size() - 1;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nth = np < 200000 ? 1 : (int)std::min<unsigned>(8u, hw);
+    std::vector<PatchHost> part(nth);
+    std::vector<int> rcs(nth, FVM_OK);
+    auto work = [&](int t) {
+        const int k0 = (int)((long long)npatch * t / nth), k1 = (int)((long long)npatch * (t + 1) / nth);
+        rcs[t] = build_patch_range(np, patch_ptr, aptr, adj, v2t_ptr, v2t, tri, v2e_ptr, v2e, enode, v2b_ptr,
+                                   v2b, bnode, iflow, split, k0, k1, part[t]);
+    };
+    if (nth == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nth; ++t) th.emplace_back(work, t);
+        for (auto &x : th) x.join();
+    }
+    for (int t = 0; t < nth; ++t)
+        if (rcs[t]) return rcs[t];
+    ph = std::move(part[0]);
+    for (int t = 1; t < nth; ++t) {
+        PatchHost &q = part[t];
+        const int off[][2] = {{PH_HALO, (int)ph.halo.size()}, {PH_TRI, (int)ph.ltri.size()},
+                              {PH_GPTR, (int)ph.gptr.size()}, {PH_GTL, (int)ph.gtl.size()},
+                              {PH_EDGE, (int)ph.ledge.size()}, {PH_BND, (int)ph.lbnd.size()},
+                              {PH_PPTR, (int)ph.pptr.size()}, {PH_PSL, (int)ph.pslot.size()}};
+        for (size_t h = 0; h < q.hdr.size(); h += kPatchHdr)
+            for (const auto &o : off) q.hdr[h + o[0]] += o[1];
+        auto cat = [](auto &a, const auto &b) { a.insert(a.end(), b.begin(), b.end()); };
+        cat(ph.hdr, q.hdr); cat(ph.halo, q.halo); cat(ph.gtri, q.gtri); cat(ph.gptr, q.gptr);
+        cat(ph.ledge_id, q.ledge_id); cat(ph.lbnd_id, q.lbnd_id); cat(ph.pptr, q.pptr);
+        cat(ph.ltri, q.ltri); cat(ph.gtl, q.gtl); cat(ph.pslot, q.pslot); cat(ph.ledge, q.ledge);
+        cat(ph.lbnd, q.lbnd); cat(ph.ledge_ds, q.ledge_ds);
+        ph.lds_max = std::max(ph.lds_max, q.lds_max);
+        ph.sa_lds_max = std::max(ph.sa_lds_max, q.sa_lds_max);
+        q = PatchHost();                          // frees the fragment
     }
     return ph.lds_max > lds_limit ? FVM_E_SIZE : FVM_OK;
 }

@@ -264,3 +264,22 @@ def test_residual_hook_leaves_iteration_state():
         hb, _ = b.iterate(10)
         np.testing.assert_array_equal(a.get_state()["prim"], b.get_state()["prim"])
         np.testing.assert_array_equal(ha, hb)
+
+
+def test_large_grid_residual_matches_oracle():
+    """1025x257 (263 168 vertices: the partition and the patch tables are
+    built on several host threads) on a non-uniform state: the residual is
+    the oracle's bit for bit, so every gather table and term order holds."""
+    from oracle.oracle import Oracle
+    g = mesh.naca0012_1025x257()
+    kw = dict(iflow=1, iflux=1, ilimit=1, mach_inf=0.5, aoa_deg=2.0, cfl=1.0, maxiter=10)
+    o = Oracle(g.coord, g.elem, g.ptype, **kw)
+    prim = o.get("prim").reshape(-1, 4).copy()
+    x, y = g.coord[:, 0], g.coord[:, 1]
+    prim[:, 0] *= 1.0 + 0.05 * np.sin(3.0 * x) * np.cos(2.0 * y)
+    prim[:, 1] += 0.05 * np.cos(5.0 * y)
+    prim[:, 3] *= 1.0 + 0.03 * np.sin(x + y)
+    o.set("prim", prim)
+    res_o = o.residu()
+    with Solver(g, Inputs.from_kwargs(**kw)) as s:
+        np.testing.assert_array_equal(s.residu(prim.ravel()), res_o)
