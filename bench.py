@@ -278,6 +278,68 @@ def serving_leg(n: int = 32, seed: int = 11, max_iters: int = 20000) -> dict:
             "saturation": sat, "load_50": run(0.5 * rate), "load_90": run(0.9 * rate)}
 
 
+def serving_full(n: int, seed: int, cap: int, rate: float, plain_every: int = 0,
+                 step_vertex_iters: float = 3e8) -> dict:
+    """The serving workload as specified: a seeded stream of n requests
+    (NACA 4-digit airfoils, max camber 0-4 %, camber position 0.2-0.6,
+    thickness 9-18 %; O-grids of 1k-1M vertices, log-uniform; M 0.3-0.85,
+    alpha -4..8 deg; 70 % Euler, 30 % SA-RANS with Re 1e6-1e7 log-uniform),
+    each solved to a 1e-6 residual drop with an iteration cap of `cap`.
+    rate <= 0: every request at t = 0, which gives the saturation rate; else
+    Poisson arrivals at that rate.  With plain_every = k, every k-th request
+    is also solved alone by the plain FP64 path (one context, fvm_gpu_iterate,
+    same cap) for the throughput comparison on the same requests."""
+    import numpy as np
+    from fvm2d_amd import serving
+    from fvm2d_amd.gpu import FvmError
+    cfg = serving.ServeConfig(max_iters=cap, minres=1e-6, step_vertex_iters=step_vertex_iters)
+    reqs = serving.poisson_arrivals(serving.workload(n, seed), rate, seed)
+    out = {"workload": f"{n} requests (seed {seed}), NACA 4-digit O-grids of 1k-1M vertices, 70/30 "
+                       f"Euler/SA-RANS (Euler split precision, SA-RANS FP64), each to a 1e-6 residual "
+                       f"drop, iteration cap {cap}",
+           "offered_per_s": rate, "vertices_total": int(sum(q.np for q in reqs))}
+    if plain_every:
+        vi, tp, cnt = 0.0, 0.0, 0
+        c64 = serving.ServeConfig(max_iters=cap, minres=1e-6, precision=0, patch_size=0)
+        for q in reqs[::plain_every]:
+            g1, geo1 = serving.prepare(q, c64)
+            with serving.create(q, g1, geo1, c64) as x:
+                t0 = time.perf_counter()
+                try:
+                    x.iterate(cap)
+                except FvmError:
+                    pass
+                tp += time.perf_counter() - t0
+                vi += x.np * x.monitor()[0]["iter"]
+                cnt += 1
+            print(f"plain FP64: {cnt} requests, {vi / tp:.3e} vertex-iterations/s", file=sys.stderr, flush=True)
+        out["plain_fp64_one_at_a_time"] = {"requests": cnt, "every": plain_every,
+                                           "vertex_iters_per_s": round(vi / tp, 1),
+                                           "solve_time_s": round(tp, 1)}
+        return out
+    srv = serving.Server(cfg)
+    res = srv.run(reqs, progress=30.0)
+    s = serving.summary(res, srv.wall)
+    lat = np.array([r.latency for r in res])
+    conv = np.array([r.latency for r in res if r.status == 1])
+    by_flow = {}
+    for flow in ("euler", "rans"):
+        rs = [r for r in res if r.flow == flow]
+        by_flow[flow] = {"requests": len(rs), "converged": sum(r.status == 1 for r in rs),
+                         "capped": sum(r.status == 2 for r in rs), "failed": sum(r.status < 0 for r in rs)}
+    out.update({"solves": s["solves"], "wall_s": s["wall_s"], "solves_per_s": round(s["solves_per_s"], 4),
+                "vertex_iters_per_s": round(s["vertex_iters_per_s"], 1),
+                "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+                "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+                "p50_converged_latency_s": round(float(np.percentile(conv, 50)), 3) if conv.size else None,
+                "p99_converged_latency_s": round(float(np.percentile(conv, 99)), 3) if conv.size else None,
+                "converged": s["converged"], "capped": s["capped"], "failed": s["failed"], "by_flow": by_flow,
+                "peak_in_flight": srv.peak_in_flight, "peak_reserved_bytes": srv.peak_reserved,
+                "engine_steps": srv.steps, "engine_phase_s": {k: round(v, 2) for k, v in srv.phase.items()},
+                "errors": srv.errors[:10]})
+    return out
+
+
 def config_legs(iters_lam: int, iters_rans: int) -> dict:
     """The viscous workloads, FP64 and split precision.
 
@@ -481,6 +543,15 @@ def main() -> int:
     ap.add_argument("--rans-converge", type=int, default=0, metavar="N",
                     help="only run the RANS batch of 8 to a 1e-8 residual drop or N iterations, "
                          "FP64 and split, and report CL/CD and their differences")
+    ap.add_argument("--serving-full", type=int, default=0, metavar="N",
+                    help="only run the serving workload as specified with N requests (see serving_full)")
+    ap.add_argument("--serving-cap", type=int, default=1500, help="iteration cap of --serving-full")
+    ap.add_argument("--serving-rate", type=float, default=0.0,
+                    help="arrival rate of --serving-full in requests/s (0: all at once, saturation)")
+    ap.add_argument("--serving-step", type=float, default=3e8,
+                    help="vertex-iterations per class and engine step of --serving-full")
+    ap.add_argument("--serving-plain-every", type=int, default=0,
+                    help="with --serving-full: only solve every k-th request alone by the plain FP64 path")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES, five times: the "
                          "reference reading for the FETCH_SIZE/WRITE_SIZE counters")
@@ -497,6 +568,10 @@ def main() -> int:
         return 0
     if a.configs:
         print(json.dumps(config_legs(*a.configs)), flush=True)
+        return 0
+    if a.serving_full:
+        print(json.dumps(serving_full(a.serving_full, 2024, a.serving_cap, a.serving_rate,
+                                      a.serving_plain_every, a.serving_step)), flush=True)
         return 0
     if a.rans_converge:
         print(json.dumps(rans_converge(a.rans_converge)), flush=True)

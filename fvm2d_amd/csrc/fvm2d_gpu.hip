@@ -1595,22 +1595,52 @@ inline DevState no_errors()
     return s;
 }
 
+// Device memory of contexts and batches is stream-ordered (hipMallocAsync /
+// hipFreeAsync on their own streams, from the device's default pool, which
+// keeps freed memory for reuse): hipFree waits for the work of every stream
+// on the device, so a solve created or destroyed while other solves' chunks
+// run would stall the host until they end.
+void keep_pool_memory()
+{
+    static std::mutex mu;
+    static bool done[64] = {false};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) { (void)hipGetLastError(); return; }
+    std::lock_guard<std::mutex> lock(mu);
+    if (done[dev]) return;
+    done[dev] = true;
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    (void)hipGetLastError();
+}
+
 template <class T>
 int dalloc(fvm_ctx *x, T **p, size_t n)
 {
     size_t b = std::max<size_t>(n, 1) * sizeof(T);
-    if (hipMalloc((void **)p, b) != hipSuccess) { *p = nullptr; return FVM_E_NOMEM; }
+    if (hipMallocAsync((void **)p, b, x->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return FVM_E_NOMEM;
+    }
     x->allocs.push_back((void *)*p);
     x->bytes += (long long)b;
     return FVM_OK;
 }
 
+// copy on the context's stream, drained before the host vector can go
 template <class T>
 int upload(fvm_ctx *x, T **p, const std::vector<T> &h)
 {
     int rc = dalloc(x, p, h.size());
     if (rc) return rc;
-    if (!h.empty()) HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!h.empty()) {
+        HIPCHK_S(x->stream, hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));
+    }
     return FVM_OK;
 }
 
@@ -1920,7 +1950,8 @@ BatchMember member_of(const fvm_ctx *x)
 int sync_self(fvm_ctx *x)
 {
     BatchMember m = member_of(x);
-    HIPCHK(hipMemcpy(x->self_mem, &m, sizeof m, hipMemcpyHostToDevice));
+    HIPCHK_S(x->stream, hipMemcpyAsync(x->self_mem, &m, sizeof m, hipMemcpyHostToDevice, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
     return FVM_OK;
 }
 
@@ -2713,6 +2744,7 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     if (x->nff < 0) x->nff = 0;
     set_consts(x);
     if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) { delete x; return FVM_E_HIP; }
+    keep_pool_memory();
 
     // LDS a workgroup may use on this device (less the static LDS), CUs and
     // LDS per CU for the automatic patch size
@@ -2901,8 +2933,11 @@ int fvm_gpu_destroy(fvm_ctx *x)
     if (x->batch) (void)fvm_batch_remove(x->batch, x);
     if (x->stream) (void)hipStreamSynchronize(x->stream);
     drop_graphs(x);
-    for (void *p : x->allocs) (void)hipFree(p);
-    if (x->stream) (void)hipStreamDestroy(x->stream);
+    if (x->stream) {
+        for (void *p : x->allocs) (void)hipFreeAsync(p, x->stream);
+        (void)hipStreamSynchronize(x->stream);
+        (void)hipStreamDestroy(x->stream);
+    }
     x->tag = 0;
     delete x;
     return FVM_OK;
@@ -2925,15 +2960,20 @@ int fvm_gpu_wall_dist(int np, const double *coord, const int *ptype, int netot, 
     hipStream_t st = nullptr;
     int rc = FVM_OK;
     auto cleanup = [&] {
-        if (st) (void)hipStreamSynchronize(st);
-        (void)hipFree(dxy); (void)hipFree(dse); (void)hipFree(dsol); (void)hipFree(dwd);
-        if (st) (void)hipStreamDestroy(st);
+        if (!st) return;
+        (void)hipStreamSynchronize(st);
+        for (void *p : {(void *)dxy, (void *)dse, (void *)dsol, (void *)dwd})
+            if (p) (void)hipFreeAsync(p, st);
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
     };
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc((void **)&dxy, sizeof(double2) * np) != hipSuccess ||
-        hipMalloc((void **)&dse, sizeof(int2) * std::max<size_t>(se.size(), 1)) != hipSuccess ||
-        hipMalloc((void **)&dsol, np) != hipSuccess ||
-        hipMalloc((void **)&dwd, sizeof(double) * np) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return FVM_E_HIP;
+    keep_pool_memory();
+    if (hipMallocAsync((void **)&dxy, sizeof(double2) * np, st) != hipSuccess ||
+        hipMallocAsync((void **)&dse, sizeof(int2) * std::max<size_t>(se.size(), 1), st) != hipSuccess ||
+        hipMallocAsync((void **)&dsol, np, st) != hipSuccess ||
+        hipMallocAsync((void **)&dwd, sizeof(double) * np, st) != hipSuccess) {
+        (void)hipGetLastError();
         cleanup();
         return FVM_E_NOMEM;
     }
@@ -3254,19 +3294,23 @@ void batch_free(fvm_batch *b)
     b->graph_n = b->graph_1 = nullptr;
     for (void *p : {(void *)b->d_mem, (void *)b->d_work, (void *)b->d_vwork, (void *)b->d_work_mon,
                     (void *)b->d_states})
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, b->stream);
     b->d_mem = nullptr; b->d_work = b->d_vwork = b->d_work_mon = nullptr; b->d_states = nullptr;
     b->nwork = b->nvwork = 0;
 }
 
 template <class T>
-int batch_upload(T **p, const std::vector<T> &h)
+int batch_upload(fvm_batch *b, T **p, const std::vector<T> &h)
 {
-    if (hipMalloc((void **)p, std::max<size_t>(h.size(), 1) * sizeof(T)) != hipSuccess) {
+    if (hipMallocAsync((void **)p, std::max<size_t>(h.size(), 1) * sizeof(T), b->stream) != hipSuccess) {
+        (void)hipGetLastError();
         *p = nullptr;
         return FVM_E_NOMEM;
     }
-    if (!h.empty()) HIPCHK(hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!h.empty()) {
+        HIPCHK_S(b->stream, hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
     return FVM_OK;
 }
 
@@ -3298,17 +3342,21 @@ int batch_rebuild(fvm_batch *b)
         work_mon = with_monitors(work, cost, (int)b->mem.size(), cus, pcu);
         order_work(work, cost, cus, pcu);
     }
-    int rc = batch_upload(&b->d_mem, m);
-    if (!rc) rc = batch_upload(&b->d_work, work);
-    if (!rc) rc = batch_upload(&b->d_work_mon, work_mon);
-    if (!rc) rc = batch_upload(&b->d_vwork, vwork);
-    if (!rc && hipMalloc((void **)&b->d_states, std::max<size_t>(m.size(), 1) * sizeof(DevState)) != hipSuccess)
+    int rc = batch_upload(b, &b->d_mem, m);
+    if (!rc) rc = batch_upload(b, &b->d_work, work);
+    if (!rc) rc = batch_upload(b, &b->d_work_mon, work_mon);
+    if (!rc) rc = batch_upload(b, &b->d_vwork, vwork);
+    if (!rc && hipMallocAsync((void **)&b->d_states, std::max<size_t>(m.size(), 1) * sizeof(DevState),
+                              b->stream) != hipSuccess) {
+        (void)hipGetLastError();
         rc = FVM_E_NOMEM;
+    }
     if (!rc && b->hs_cap < (int)m.size()) {
+        // grows rarely (hipHostFree waits for the whole device)
         if (b->hs) (void)hipHostFree(b->hs);
         b->hs = nullptr;
         b->hs_cap = 0;
-        const int cap = std::max<int>(16, (int)m.size() * 2);
+        const int cap = std::max<int>(1024, (int)m.size() * 2);
         if (hipHostMalloc((void **)&b->hs, sizeof(DevState) * cap, hipHostMallocDefault) != hipSuccess) {
             b->hs = nullptr;
             rc = FVM_E_NOMEM;
@@ -3393,6 +3441,7 @@ int fvm_batch_create(fvm_batch **out)
     *out = nullptr;
     fvm_batch *b = new fvm_batch();
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return FVM_E_HIP; }
+    keep_pool_memory();
     *out = b;
     return FVM_OK;
 }
@@ -3484,6 +3533,16 @@ int fvm_batch_launch(fvm_batch *b, int n)
     return FVM_OK;
 }
 
+int fvm_batch_query(fvm_batch *b)
+{
+    if (!batch_ok(b)) return FVM_E_INPUT;
+    if (!b->pending) return 1;
+    const hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return FVM_E_HIP;
+}
+
 int fvm_batch_wait(fvm_batch *b, int *status)
 {
     if (!batch_ok(b)) return FVM_E_INPUT;
@@ -3520,6 +3579,7 @@ int fvm_batch_destroy(fvm_batch *b)
     for (fvm_ctx *x : b->mem) x->batch = nullptr;
     b->mem.clear();
     batch_free(b);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->hs) (void)hipHostFree(b->hs);
     b->hs = nullptr;
     if (b->stream) (void)hipStreamDestroy(b->stream);

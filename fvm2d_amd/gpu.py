@@ -106,6 +106,8 @@ def lib():
     L.fvm_batch_iterate.argtypes = [vp, ci, vp]
     L.fvm_batch_launch.restype = ci
     L.fvm_batch_launch.argtypes = [vp, ci]
+    L.fvm_batch_query.restype = ci
+    L.fvm_batch_query.argtypes = [vp]
     L.fvm_batch_wait.restype = ci
     L.fvm_batch_wait.argtypes = [vp, vp]
     L.fvm_batch_destroy.restype = ci

@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import sys
 import time
 
 import numpy as np
@@ -125,9 +126,11 @@ class ServeConfig:
     rans_precision: int = 0
     minres: float = 1e-6              # residual drop that ends a solve
     max_iters: int = 2000             # iteration cap of a solve
-    step_vertex_iters: float = 2e7     # work of one engine step per class (sets its chunk)
+    # work of one engine step per class (sets its chunk): long enough that the
+    # host work done meanwhile (meshing, preprocessing, context creation, up
+    # to about a second for a million-vertex request) finds the GPU busy
+    step_vertex_iters: float = 1e8
     size_classes: tuple = (16384, 131072)   # vertex bounds of the mesh-size classes
-    prep_slice: float = 0.02          # seconds of host preparation per engine step
     cfl: float = 1.0
     first_spacing: float = 1e-3       # wall spacing of the O-grids (config 0 construction)
     # vertices per patch of a served solve: a batch fills the GPU with many
@@ -180,11 +183,11 @@ class Server:
 
     One thread does everything, in steps: launch a chunk of iterations on
     every class's batch (each batch on its own stream, so the chunks run at
-    the same time); while they run, prepare the requests that have arrived
-    (host meshing and preprocessing, up to prep_slice seconds a step) and
-    admit those the HBM budget holds (device wall distance, context
-    creation); then wait, retire the members that stopped and let the new
-    ones join."""
+    the same time); while any chunk still runs, do host work one request at a
+    time: admit the oldest waiting request if the HBM budget holds it (device
+    wall distance, context creation), else prepare the oldest arrived one
+    (host meshing and preprocessing); then wait, retire the members that
+    stopped and let the new ones join."""
 
     def __init__(self, cfg: ServeConfig | None = None):
         self.cfg = cfg or ServeConfig()
@@ -198,6 +201,8 @@ class Server:
         self.peak_reserved = 0             # most bytes admitted at once
         self.steps = 0
         self.wall = 0.0
+        # wall seconds of the engine thread by phase
+        self.phase = dict(launch=0.0, prepare=0.0, admit=0.0, wait=0.0, retire=0.0, join=0.0, idle=0.0)
 
     def now(self) -> float:
         return time.perf_counter() - self.t0
@@ -215,26 +220,13 @@ class Server:
         k = int(round(self.cfg.step_vertex_iters / max(nv, 1) / 8.0)) * 8
         return max(8, min(512, k))
 
-    def _prepare_arrived(self, arrivals: list) -> None:
-        """Host preparation of arrived requests, oldest first, for at most
-        prep_slice seconds (at least one request)."""
-        t_end = self.now() + self.cfg.prep_slice
-        while arrivals and arrivals[0].arrival <= self.now():
-            r = arrivals.pop(0)
-            try:
-                g, geo = prepare(r, self.cfg)
-                self.waiting.append((r, g, geo, footprint(geo, r, self.cfg), self.now()))
-            except Exception as e:  # noqa: BLE001 - recorded, the run goes on
-                self.errors.append((r.rid, repr(e)))
-            if self.now() > t_end:
-                break
-
-    def _admit(self) -> list:
-        """Admits the waiting requests the budget holds (first come first
-        served) and creates their contexts; returns them, to join their
-        batches after the running chunk."""
-        ready = []
-        while self.waiting and self.waiting[0][3] <= self.budget:
+    def _host_unit(self, arrivals: list):
+        """One unit of host work: admit the oldest waiting request if the
+        budget holds its footprint (returns its entry, to join its batch
+        after the running chunk), else prepare the oldest arrived request.
+        Returns (did_work, entry or None)."""
+        t0 = time.perf_counter()
+        if self.waiting and self.waiting[0][3] <= self.budget:
             r, g, geo, fp, t_prep = self.waiting.pop(0)
             self.budget -= fp
             self.in_flight += 1
@@ -243,13 +235,24 @@ class Server:
             t_adm = self.now()
             try:
                 s = create(r, g, geo, self.cfg)
-            except Exception as e:  # noqa: BLE001
+            except Exception as e:  # noqa: BLE001 - recorded, the run goes on
                 self.errors.append((r.rid, repr(e)))
                 self.budget += fp
                 self.in_flight -= 1
-                continue
-            ready.append(((r.flow, self.size_class(s.np)), [r, s, fp, t_prep, t_adm, 0.0]))
-        return ready
+                self.phase["admit"] += time.perf_counter() - t0
+                return True, None
+            self.phase["admit"] += time.perf_counter() - t0
+            return True, ((r.flow, self.size_class(s.np)), [r, s, fp, t_prep, t_adm, 0.0])
+        if arrivals and arrivals[0].arrival <= self.now():
+            r = arrivals.pop(0)
+            try:
+                g, geo = prepare(r, self.cfg)
+                self.waiting.append((r, g, geo, footprint(geo, r, self.cfg), self.now()))
+            except Exception as e:  # noqa: BLE001
+                self.errors.append((r.rid, repr(e)))
+            self.phase["prepare"] += time.perf_counter() - t0
+            return True, None
+        return False, None
 
     def _join(self, classes, ready):
         for key, entry in ready:
@@ -278,27 +281,54 @@ class Server:
             self.in_flight -= 1
         members[:] = keep
 
-    def run(self, reqs: list[Request], timeout: float | None = None) -> list[Result]:
+    def run(self, reqs: list[Request], timeout: float | None = None,
+            progress: float | None = None) -> list[Result]:
+        """Serves reqs to completion; with `progress` (seconds), prints a line
+        on stderr that often (requests done, in flight, waiting)."""
         n = len(reqs)
         arrivals = sorted(reqs, key=lambda r: r.arrival)
         classes: dict = {}
         self.t0 = time.perf_counter()
+        t_print = progress
         try:
             while len(self.results) + len(self.errors) < n:
                 if timeout is not None and self.now() > timeout:
                     raise TimeoutError(f"serving run over {timeout} s: {len(self.results)} of {n} done")
+                if progress is not None and self.now() > t_print:
+                    t_print += progress
+                    print(f"serving: {self.now():.0f} s, {len(self.results)} done, {self.in_flight} in flight, "
+                          f"{len(self.waiting)} waiting, {len(arrivals)} not arrived, {len(self.errors)} errors",
+                          file=sys.stderr, flush=True)
                 live = [(b, m) for b, m in classes.values() if m]
+                tp = time.perf_counter()
                 for b, m in live:
                     b.launch(self._chunk(m))
-                self._prepare_arrived(arrivals)
-                ready = self._admit()
+                self.phase["launch"] += time.perf_counter() - tp
+                # host work while the chunks run (at least one unit a step);
+                # with nothing running, until a request is admitted
+                ready = []
+                while True:
+                    did, entry = self._host_unit(arrivals)
+                    if entry is not None:
+                        ready.append(entry)
+                    if not did or (live and all(b.done() for b, _ in live)) or (not live and ready):
+                        break
                 for b, m in live:
-                    self._retire(b, m, b.wait())
+                    tp = time.perf_counter()
+                    st = b.wait()
+                    tq = time.perf_counter()
+                    self._retire(b, m, st)
+                    self.phase["wait"] += tq - tp
+                    self.phase["retire"] += time.perf_counter() - tq
+                tp = time.perf_counter()
                 self._join(classes, ready)
+                self.phase["join"] += time.perf_counter() - tp
                 if live:
                     self.steps += 1
                 elif not ready and not self.waiting and arrivals:
+                    tp = time.perf_counter()
                     time.sleep(max(0.0, min(arrivals[0].arrival - self.now(), 0.01)))
+                    self.phase["idle"] += time.perf_counter() - tp
         finally:
             for b, m in classes.values():
                 for entry in m:

@@ -348,6 +348,13 @@ class Batch:
         if rc != 0:
             raise FvmError(rc, 0, "fvm_batch_launch")
 
+    def done(self) -> bool:
+        """True when the launched iterations have finished (never blocks)."""
+        rc = self.L.fvm_batch_query(self.h)
+        if rc < 0:
+            raise FvmError(rc, 0, "fvm_batch_query")
+        return rc == 1
+
     def wait(self) -> list[int]:
         """Wait for the launched iterations; statuses in member order."""
         st = np.zeros(max(len(self.members), 1), dtype=np.int32)

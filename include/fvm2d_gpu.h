@@ -216,6 +216,10 @@ int fvm_batch_iterate(fvm_batch *batch, int n, int *status);
  * remove and iterate wait for a launched chunk first. */
 int fvm_batch_launch(fvm_batch *batch, int n);
 int fvm_batch_wait(fvm_batch *batch, int *status);
+/* 1 when the launched chunk has finished (or none is pending), 0 while it
+ * runs; never blocks.  No counterpart in the reference: lets a serving loop
+ * fill the time a chunk runs with host work. */
+int fvm_batch_query(fvm_batch *batch);
 int fvm_batch_destroy(fvm_batch *batch);
 
 /* Monitors of the last iteration (src/main.f:63 RES.DAT row): mon[6] =

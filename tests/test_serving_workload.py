@@ -73,6 +73,12 @@ class ModelBatch:
     def launch(self, n):
         assert self.n is None and n % 8 == 0
         self.n = n
+        self.polls = 0
+
+    def done(self):
+        # the chunk "runs" for two polls, so host work interleaves with it
+        self.polls += 1
+        return self.polls > 2
 
     def wait(self):
         time.sleep(0.001)                       # a step takes time
