@@ -3,7 +3,9 @@ C
 C TEST INFRASTRUCTURE ONLY.  Linked by oracle/Makefile against the
 C reference sources compiled where they lie (src/*.f); never shipped.
 C It replaces main.f's loop (main.f:42-76) so that it can write full
-C precision arrays (the reference's own files carry 7 digits, main.f:63):
+C precision arrays (the reference's own files carry 7 digits, main.f:63),
+C and ends with the reference's write_result and finalize (WALL.DAT, SOL,
+C PARAM.DAT in the output directory):
 C   argv(1) deck, argv(2) number of iterations, argv(3) output directory.
 C The sa_model call passes nut as the 7th argument; main.f:54-55 omits it.
       program refdump
@@ -122,6 +124,11 @@ C The sa_model call passes nut as the 7th argument; main.f:54-55 omits it.
          iter = iter + 1
       enddo
       close(31)
+
+C     the reference's own output at the end of the run (main.f:79-84):
+C     WALL.DAT and PARAM.DAT, with the last stage's gradients and viscosity
+      call write_result(prim, nut, mu, qx, qy, coord, dsb)
+      call finalize(prim, nut, coord)
 
       call dumpd(od, 'primN', prim, nvar*np)
       call dumpd(od, 'nutN', nut, np)

@@ -4,8 +4,9 @@ Runs oracle/_ref/refdump (the reference's own Fortran routines compiled with
 flang, see oracle/Makefile, linked to the dump harness oracle/ref_dump.f) on
 seeded NACA O-grids and stores what it writes -- dual-mesh geometry, the
 residual at the initial and final states, states after the first and last
-iteration, and the full-precision (iter, fres, iresi, fresi, cl, cd)
-history -- as compressed .npz fixtures.  Needs /root/reference, so it only
+iteration, the full-precision (iter, fres, iresi, fresi, cl, cd) history,
+and the text of the WALL.DAT and PARAM.DAT the reference writes at the end --
+as compressed .npz fixtures.  Needs /root/reference, so it only
 runs in the development container:
 
     make -C oracle ref && python tests/golden/make_golden.py
@@ -70,10 +71,16 @@ def run_case(name: str, case: dict) -> dict:
             dt = np.int32 if key in INT_NAMES else np.float64
             out[key] = np.fromfile(os.path.join(td, fn), dtype=dt)
         out["hist"] = out.get("hist", np.zeros(0)).reshape(-1, 6)
+        # the reference's WALL.DAT and PARAM.DAT text (result.f:92-118,
+        # finalize.f:26-61), as bytes
+        for fn, key in (("WALL.DAT", "wall_text"), ("PARAM.DAT", "param_text")):
+            path = os.path.join(td, fn)
+            if os.path.exists(path):
+                out[key] = np.frombuffer(open(path, "rb").read(), dtype=np.uint8)
     return out
 
 
-def main(names=None):
+def main(names=None, out_dir=OUT):
     if not os.path.exists(REFDUMP):
         raise SystemExit("oracle/_ref/refdump is not built (make -C oracle ref)")
     for name, case in CASES.items():
@@ -82,7 +89,7 @@ def main(names=None):
         data = run_case(name, case)
         if not case.get("geometry", True):
             data = {k: v for k, v in data.items() if k not in GEOMETRY}
-        path = os.path.join(OUT, f"{name}.npz")
+        path = os.path.join(out_dir, f"{name}.npz")
         np.savez_compressed(path, **data)
         print(f"{name}: {len(data)} arrays, {os.path.getsize(path) / 1024:.0f} KiB, "
               f"{data['hist'].shape[0]} iterations")
