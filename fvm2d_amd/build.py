@@ -69,7 +69,9 @@ def build_host(verbose: bool = False) -> str:
     objs = []
     for src in ("fvm_gpu_mod.f90", "fvm2d_gpu.f90"):
         obj = os.path.join(mod, src.replace(".f90", ".o"))
-        _run([FLANG, "-O2", "-J", mod, "-c", os.path.join(FORTRAN, src), "-o", obj])
+        # no fused multiply-add: the host's own arithmetic (the state rebuilt
+        # from SOL, WALL.DAT, PARAM.DAT) rounds as the reference's build does
+        _run([FLANG, "-O2", "-ffp-contract=off", "-J", mod, "-c", os.path.join(FORTRAN, src), "-o", obj])
         objs.append(obj)
     tmp = HOST_EXE + ".tmp"
     _run([FLANG, "-o", tmp, *objs, "-L", OUTDIR, "-lfvm2d_gpu", "-Wl,-rpath,$ORIGIN"])

@@ -7,7 +7,13 @@
 ! iterations go through the C-ABI of libfvm2d_gpu (fvm_gpu_mod); the loop
 ! condition of src/main.f:42-76 is evaluated on the device.
 !
-!     fvm2d_gpu <deck> [<precision 0|1>]
+!     fvm2d_gpu <deck> [<precision 0|1> [residu | batch=K]]
+!
+! residu: writes RESIDU.DAT, the residual of the start state through the
+! parity hook fvm_gpu_residu (src/residu.f:2-3), res(4,np) at full
+! precision, and stops.  batch=K: K solves of the deck at AOA, AOA+2, ...,
+! AOA+2(K-1) deg advanced together (fvm_batch_*), each member's last RES.DAT
+! row in BATCH.DAT at full precision.
 !
 ! Plots (gnuplot, mayavi, vigie files) are not written.
 program fvm2d_gpu
@@ -22,8 +28,10 @@ program fvm2d_gpu
   integer :: istart, iflow, iterlast, maxiter, saveinterval, niso, iflux, ilimit, vortex, &
              cell_type
   real(c_double) :: mach_inf, aoa_deg, rey, cfl, minres, xref, yref
-  character(len=256) :: deckfile, gridfile, arg
-  integer :: precision
+  character(len=256) :: deckfile, gridfile, arg, mode
+  integer :: precision, nbatch
+  ! range of the solution at the last write_result (src/result.f:16-48), for PARAM.DAT
+  real(c_double) :: range_lo(7), range_hi(7)
 
   ! grid and geometry (the reference's layout)
   integer(c_int) :: np, nt, me
@@ -50,6 +58,9 @@ program fvm2d_gpu
   precision = 0
   call get_command_argument(2, arg)
   if (len_trim(arg) > 0) read (arg, *) precision
+  call get_command_argument(3, mode)
+  nbatch = 0
+  if (mode(1:6) == 'batch=') read (mode(7:), *) nbatch
   call read_deck()
 
   ! ---- grid (src/read_grid.f:19-39) ----
@@ -136,6 +147,17 @@ program fvm2d_gpu
     end if
   else
     print *, 'Initializing solution to free stream values'
+  end if
+
+  if (trim(mode) == 'residu') then
+    call write_residu()
+    rc = fvm_gpu_destroy(ctx)
+    stop
+  end if
+  if (nbatch > 0) then
+    call run_batch()
+    rc = fvm_gpu_destroy(ctx)
+    stop
   end if
 
   ! ---- iterations (src/main.f:36-76) ----
@@ -243,51 +265,138 @@ contains
     end select
   end subroutine stop_message
 
-  ! WALL.DAT (src/result.f:92-118): x, Cp and, with viscosity, Cf, the first
-  ! off-wall distance and y+ at every solid point
+  ! RESIDU.DAT: the residual of the current state through the hook
+  subroutine write_residu()
+    real(c_double), allocatable :: res(:,:)
+    integer :: ierr, ii
+    allocate (res(4, np))
+    ierr = fvm_gpu_get_state(ctx, prim, nut, qx, qy, mu)
+    ierr = fvm_gpu_residu(ctx, prim, nut, res)
+    if (ierr /= FVM_OK) then
+      call stop_message(ierr, 0)
+      stop
+    end if
+    open (unit=20, file='RESIDU.DAT', status='replace')
+    do ii = 1, np
+      write (20, '(4es26.17e3)') res(:, ii)
+    end do
+    close (20)
+  end subroutine write_residu
+
+  ! K solves of this deck, alpha stepped by 2 deg, as one batch
+  subroutine run_batch()
+    type(c_ptr), allocatable :: member(:)
+    type(c_ptr) :: batch
+    type(fvm_params) :: pk
+    integer(c_int), allocatable :: st(:)
+    integer(c_int) :: ierr, left
+    integer :: kk
+    allocate (member(nbatch), st(nbatch))
+    ierr = fvm_batch_create(batch)
+    do kk = 1, nbatch
+      pk = par
+      pk%aoa_deg = aoa_deg + 2.0d0 * (kk - 1)
+      ierr = fvm_gpu_create(pk, mesh, member(kk))
+      if (ierr == FVM_OK) ierr = fvm_batch_add(batch, member(kk))
+      if (ierr /= FVM_OK) then
+        print *, 'batch member ', kk, ': ', trim(status_text(ierr))
+        stop
+      end if
+    end do
+    left = maxiter
+    do while (left > 0)
+      ierr = fvm_batch_iterate(batch, min(left, 4096), st)
+      if (ierr /= FVM_OK) then
+        print *, 'fvm_batch_iterate: ', trim(status_text(ierr))
+        stop
+      end if
+      left = left - min(left, 4096)
+      if (all(st /= FVM_OK)) exit
+    end do
+    open (unit=20, file='BATCH.DAT', status='replace')
+    do kk = 1, nbatch
+      ierr = fvm_gpu_monitor(member(kk), mon)
+      write (20, '(i4,i4,es26.17e3,i8,es26.17e3,i8,3es26.17e3)') kk, ierr, aoa_deg + 2.0d0 * (kk - 1), &
+          nint(mon(1)), mon(2), nint(mon(3)), mon(4), mon(5), mon(6)
+    end do
+    close (20)
+    ierr = fvm_batch_destroy(batch)
+    do kk = 1, nbatch
+      ierr = fvm_gpu_destroy(member(kk))
+    end do
+  end subroutine run_batch
+
+  ! Wall quantities at solid point is (WALL.DAT columns, src/result.f:92-118):
+  ! the pressure coefficient and, with viscosity, the skin friction from the
+  ! wall shear mu d(u.t)/dn (t the wall tangent, n its normal from dsb), the
+  ! off-wall distance wd1 and y+ = rho u_tau wd1 / mu.
+  subroutine wall_row(ii, is, row, ncol)
+    integer, intent(in) :: ii, is
+    real(c_double), intent(out) :: row(5)
+    integer, intent(out) :: ncol
+    real(c_double) :: qdyn, n(2), dudn(2), shear, utau
+    qdyn = 0.5d0 * r_inf * q_inf**2
+    row = 0.0d0
+    row(1) = coord(1, is)
+    row(2) = (p_inf - prim(4, is)) / qdyn
+    ncol = 2
+    if (iflow == INVISCID) return
+    n = dsb(:, is) / sqrt(dsb(1, is)**2 + dsb(2, is)**2)
+    ! normal derivative of the velocity, then its tangential part (t = (n2, -n1))
+    dudn(1) = qx(2, is) * n(1) + qy(2, is) * n(2)
+    dudn(2) = qx(3, is) * n(1) + qy(3, is) * n(2)
+    shear = mu(is) * (n(2) * dudn(1) - n(1) * dudn(2))
+    utau = sqrt(abs(shear) / prim(1, is))
+    row(3) = shear / qdyn
+    row(4) = wd1(ii)
+    row(5) = prim(1, is) * utau * wd1(ii) / mu(is)
+    ncol = 5
+  end subroutine wall_row
+
+  ! WALL.DAT at every SaveInt and at the end, and the range of the solution
+  ! that PARAM.DAT reports
   subroutine write_result()
-    integer :: ii, is, ierr
-    real(c_double) :: cp, ux, uy, vx, vy, nl, nx, ny, tx, ty, tw, cf, us, ys
+    integer :: ii, ncol, ierr
+    real(c_double) :: row(5), var(7), speed2
     ierr = fvm_gpu_get_state(ctx, prim, nut, qx, qy, mu)
     ierr = fvm_gpu_monitor(ctx, mon)
     print *, 'Iterations =', nint(mon(1)), ' Cl, Cd =', mon(5), mon(6), ' Residue =', mon(2)
+    ! density, pressure, Mach number, x and y velocity, log10 of the entropy
+    ! ratio s/s_inf, nu_tilde
+    range_lo = 1.0d10
+    range_hi = -1.0d10
+    do ii = 1, np
+      speed2 = prim(2, ii)**2 + prim(3, ii)**2
+      var = [prim(1, ii), prim(4, ii), sqrt(speed2 * prim(1, ii) / (GAMMA * prim(4, ii))), prim(2, ii), &
+             prim(3, ii), log10(prim(4, ii) / prim(1, ii)**GAMMA / (p_inf / r_inf**GAMMA)), nut(ii)]
+      range_lo = min(range_lo, var)
+      range_hi = max(range_hi, var)
+    end do
     open (unit=10, file='WALL.DAT', status='replace')
     do ii = 1, dims(5)
-      is = spts(ii)
-      cp = -(prim(4, is) - p_inf) / (0.5d0 * r_inf * q_inf**2)
-      if (iflow /= INVISCID) then
-        ux = qx(2, is); uy = qy(2, is); vx = qx(3, is); vy = qy(3, is)
-        nl = sqrt(dsb(1, is)**2 + dsb(2, is)**2)
-        nx = dsb(1, is) / nl
-        ny = dsb(2, is) / nl
-        tx = ny
-        ty = -nx
-        tw = mu(is) * ((ux * tx + vx * ty) * nx + (uy * tx + vy * ty) * ny)
-        cf = tw / (0.5d0 * r_inf * q_inf**2)
-        us = sqrt(abs(tw) / prim(1, is))
-        ys = prim(1, is) * us * wd1(ii) / mu(is)
-        write (10, *) coord(1, is), cp, cf, wd1(ii), ys
-      else
-        write (10, *) coord(1, is), cp
-      end if
+      call wall_row(ii, spts(ii), row, ncol)
+      write (10, *) row(1:ncol)
     end do
     close (10)
   end subroutine write_result
 
-  ! SOL in conserved variables and PARAM.DAT (src/finalize.f:14-59)
+  ! SOL in conserved variables and PARAM.DAT (src/finalize.f:14-61)
   subroutine finalize()
-    integer :: ierr, ii
-    real(c_double) :: v1, v2, v3, v4, v5
+    integer :: ierr, ii, ip
+    real(c_double) :: v(5)
+    real :: cpu(2), total
+    character(len=20), parameter :: what(7) = [character(len=20) :: 'density   =', 'pressure  =', &
+        'mach no   =', 'x vel     =', 'y vel     =', 'entropy   =', 'viscosity =']
     ierr = fvm_gpu_get_state(ctx, prim, nut, qx, qy, mu)
     ierr = fvm_gpu_monitor(ctx, mon)
     open (unit=20, file='SOL', status='replace')
     do ii = 1, np
-      v1 = prim(1, ii)
-      v2 = v1 * prim(2, ii)
-      v3 = v1 * prim(3, ii)
-      v4 = prim(4, ii) / GAMMA1 + 0.5d0 * prim(1, ii) * (prim(2, ii)**2 + prim(3, ii)**2)
-      v5 = nut(ii)
-      write (20, *) v1, v2, v3, v4, v5
+      v(1) = prim(1, ii)
+      v(2) = v(1) * prim(2, ii)
+      v(3) = v(1) * prim(3, ii)
+      v(4) = prim(4, ii) / GAMMA1 + 0.5d0 * prim(1, ii) * (prim(2, ii)**2 + prim(3, ii)**2)
+      v(5) = nut(ii)
+      write (20, *) v(1), v(2), v(3), v(4), v(5)
     end do
     close (20)
     open (unit=20, file='PARAM.DAT', status='replace')
@@ -301,6 +410,23 @@ contains
     write (20, '(" Cd                =",f12.6)') mon(6)
     write (20, '(" L2 residue        =",f12.6)') log10(mon(2))
     write (20, '(" Linf residue      =",e16.6)') mon(4)
+    ip = nint(mon(3))
+    if (ip >= 1 .and. ip <= np) then
+      write (20, '(" Linf point        =",f12.6,f12.6,i8,i4)') coord(1, ip), coord(2, ip), ip, ptype(ip)
+    end if
+    do ii = 1, 7
+      if (ii < 7) then
+        write (20, '(" Minimum ",a,f12.6)') what(ii)(1:11), range_lo(ii)
+        write (20, '(" Maximum ",a,f12.6)') what(ii)(1:11), range_hi(ii)
+      else
+        write (20, '(" Minimum ",a,e16.6)') what(ii)(1:11), range_lo(ii)
+        write (20, '(" Maximum ",a,e16.6)') what(ii)(1:11), range_hi(ii)
+      end if
+    end do
+    total = etime(cpu)
+    write (20, '(" Elapsed time      =",f12.6, " min")') total / 60.0
+    write (20, '(" User    time      =",f12.6, " min")') cpu(1) / 60.0
+    write (20, '(" System  time      =",f12.6, " min")') cpu(2) / 60.0
     close (20)
   end subroutine finalize
 

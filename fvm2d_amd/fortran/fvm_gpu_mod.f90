@@ -9,7 +9,8 @@ module fvm_gpu_mod
   integer(c_int), parameter :: FVM_OK = 0, FVM_CONVERGED = 1, FVM_ITER_LIMIT = 2
   integer(c_int), parameter :: FVM_E_DEGENERATE = -1, FVM_E_AREA = -2, FVM_E_ORPHAN_EDGE = -3
   integer(c_int), parameter :: FVM_E_NEG_SOUND = -4, FVM_E_NONPOSITIVE = -5
-  integer(c_int), parameter :: FVM_E_ZERO_RES1 = -6, FVM_E_INPUT = -7
+  integer(c_int), parameter :: FVM_E_ZERO_RES1 = -6, FVM_E_INPUT = -7, FVM_E_SIZE = -8
+  integer(c_int), parameter :: FVM_E_HIP = -10, FVM_E_NOMEM = -11
 
   ! fvm_params: the COMMON values of the deck (src/read_input.f:20-66)
   type, bind(C) :: fvm_params
@@ -81,6 +82,97 @@ module fvm_gpu_mod
       integer(c_int), value :: n
       real(c_double), intent(out) :: hist(*)
       integer(c_int), intent(out) :: ndone, err_index
+    end function
+
+    ! residu(...) of src/residu.f:2-3 on the given state: res(4,np) in the
+    ! caller's order (the parity hook; the iteration state is left as it was)
+    integer(c_int) function fvm_gpu_residu(ctx, prim, nut, res) bind(C, name="fvm_gpu_residu")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: prim(*), nut(*)
+      real(c_double), intent(out) :: res(*)
+    end function
+
+    ! COMMON /liftdrag/ cl, read by the farfield vortex correction
+    ! (src/farfield_flux.f:28-44)
+    integer(c_int) function fvm_gpu_set_cl(ctx, cl) bind(C, name="fvm_gpu_set_cl")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: cl
+    end function
+
+    integer(c_int) function fvm_gpu_tune(ctx, variant, vertices_per_block, xcd_remap) &
+        bind(C, name="fvm_gpu_tune")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: variant, vertices_per_block, xcd_remap
+    end function
+
+    integer(c_int) function fvm_gpu_layout_info(ctx, info) bind(C, name="fvm_gpu_layout_info")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long), intent(out) :: info(*)
+    end function
+
+    integer(c_long_long) function fvm_gpu_footprint(ctx) bind(C, name="fvm_gpu_footprint")
+      import :: c_long_long, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_long_long) function fvm_gpu_footprint_estimate(np, nt, netot, iflow, precision) &
+        bind(C, name="fvm_gpu_footprint_estimate")
+      import :: c_long_long, c_int
+      integer(c_int), value :: np, nt, netot, iflow, precision
+    end function
+
+    ! batches: many solves of one kind advanced by one launch per RK stage
+    integer(c_int) function fvm_batch_create(batch) bind(C, name="fvm_batch_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: batch
+    end function
+
+    integer(c_int) function fvm_batch_add(batch, ctx) bind(C, name="fvm_batch_add")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch, ctx
+    end function
+
+    integer(c_int) function fvm_batch_remove(batch, ctx) bind(C, name="fvm_batch_remove")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch, ctx
+    end function
+
+    integer(c_int) function fvm_batch_size(batch) bind(C, name="fvm_batch_size")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
+    integer(c_int) function fvm_batch_iterate(batch, n, status) bind(C, name="fvm_batch_iterate")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: n
+      integer(c_int), intent(out) :: status(*)
+    end function
+
+    integer(c_int) function fvm_batch_launch(batch, n) bind(C, name="fvm_batch_launch")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: n
+    end function
+
+    integer(c_int) function fvm_batch_query(batch) bind(C, name="fvm_batch_query")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
+    integer(c_int) function fvm_batch_wait(batch, status) bind(C, name="fvm_batch_wait")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), intent(out) :: status(*)
+    end function
+
+    integer(c_int) function fvm_batch_destroy(batch) bind(C, name="fvm_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
     end function
 
     integer(c_int) function fvm_gpu_monitor(ctx, mon) bind(C, name="fvm_gpu_monitor")

@@ -25,7 +25,7 @@ def conserved(prim, nut):
     return np.stack([r, r * u, r * v, p / GAMMA1 + 0.5 * r * (u**2 + v**2), nut], axis=1)
 
 
-def run_host(tmp_path, name, save_every=None, precision=0):
+def run_host(tmp_path, name, save_every=None, precision=0, mode=""):
     case = CASES[name]
     d = dict(case["deck"])
     d["MaxIter"] = case["iters"]
@@ -35,10 +35,12 @@ def run_host(tmp_path, name, save_every=None, precision=0):
     (tmp_path / "fvm.inp").write_text(deck_text(d))
     fx = fixture(name)
     if d["Start"] == 2:
-        write_sol(str(tmp_path / "SOL"), fx["prim0"], fx["nut0"])
+        # the SOL the reference read (make_golden.py writes it from this state)
+        src = fixture(case["sol_from"])
+        write_sol(str(tmp_path / "SOL"), src["primN"], src["nutN"])
     assert os.path.exists(build.HOST_EXE), "build the host program first (python -m fvm2d_amd.build)"
-    r = subprocess.run([build.HOST_EXE, "fvm.inp", str(precision)], cwd=tmp_path,
-                       capture_output=True, text=True, timeout=120)
+    args = [build.HOST_EXE, "fvm.inp", str(precision)] + ([mode] if mode else [])
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     return fx, r.stdout
 
@@ -108,7 +110,20 @@ def test_host_program_matches_reference(tmp_path, name):
     cp = -(prim[spts, 3] - p_inf) / 0.5
     assert np.abs(wall[:, 1] - cp).max() <= 1e-8 * max(1.0, np.abs(cp).max())
     assert "Stopping execution" in out
-    assert (tmp_path / "PARAM.DAT").exists()
+    # every WALL.DAT column against the reference's own file: x and Cp to the
+    # bit (the same state and operations); Cf, the off-wall distance and y+
+    # are the same quantities written in this program's own terms, so their
+    # last bits may differ
+    ref_wall = np.array([float(v) for v in bytes(fx["wall_text"]).decode().split()]).reshape(-1, ncol)
+    np.testing.assert_array_equal(wall[:, :2], ref_wall[:, :2])
+    if ncol == 5:
+        assert np.abs(wall[:, 2:] - ref_wall[:, 2:]).max() <= 1e-12 * np.abs(ref_wall[:, 2:]).max()
+    # PARAM.DAT line for line against the reference's, but the three CPU times
+    got = (tmp_path / "PARAM.DAT").read_text().splitlines()
+    want = bytes(fx["param_text"]).decode().splitlines()
+    assert len(got) == len(want)
+    assert got[:-3] == want[:-3]
+    assert [g.split("=")[0] for g in got[-3:]] == [w.split("=")[0] for w in want[-3:]]
 
 
 def test_host_program_restarts_from_its_own_sol(tmp_path):
@@ -131,3 +146,42 @@ def test_host_program_restarts_from_its_own_sol(tmp_path):
     assert np.abs(two - one).max() <= 1e-9 * np.abs(one).max()
     res = read_res(tmp_path / "RES.DAT")
     assert res[0, 0] == 50 and res[-1, 0] == 99
+
+
+def read_residu(path):
+    return np.array([float(x) for x in open(path).read().split()])
+
+
+@pytest.mark.parametrize("name", ["euler_roe_lim_129x33", "restart_kfvs_129x33", "laminar_roe_97x25",
+                                  "rans_roe_97x25", "euler_roe_barth_65x17"])
+def test_fortran_residual_hook_matches_reference(tmp_path, name):
+    """fvm_gpu_residu called from Fortran (the replacement of src/residu.f:2-3)
+    on the start state: res0 of the reference, bit for bit (KFVS: 1e-11, device
+    erf/exp)."""
+    fx, _ = run_host(tmp_path, name, mode="residu")
+    res = read_residu(tmp_path / "RESIDU.DAT")
+    if CASES[name]["deck"]["Flux"] == 2:
+        assert np.abs(res - fx["res0"]).max() <= 1e-11 * np.abs(fx["res0"]).max()
+    else:
+        np.testing.assert_array_equal(res, fx["res0"])
+
+
+def test_fortran_batch_matches_solves_alone(tmp_path):
+    """fvm_batch_* from Fortran: three angles of a deck advanced together give
+    each solve's own monitors, bit for bit."""
+    from fvm2d_amd.solver import Inputs, Solver
+    from tests.helpers import case_kwargs
+    name = "euler_roe_transonic_65x17"
+    run_host(tmp_path, name, mode="batch=3")
+    rows = np.loadtxt(tmp_path / "BATCH.DAT")
+    assert rows.shape == (3, 9)
+    aoa0 = CASES[name]["deck"]["AOA"]
+    for k, row in enumerate(rows):
+        kw = case_kwargs(name, aoa_deg=aoa0 + 2.0 * k, maxiter=CASES[name]["iters"])
+        with Solver(grid(name), Inputs.from_kwargs(**kw)) as s:
+            s.iterate(CASES[name]["iters"])
+            mon, status = s.monitor()
+        assert row[1] == status
+        assert row[2] == aoa0 + 2.0 * k
+        assert (int(row[3]), row[4], int(row[5]), row[6], row[7], row[8]) == \
+            (mon["iter"], mon["fres"], mon["iresi"], mon["fresi"], mon["cl"], mon["cd"])
