@@ -229,7 +229,7 @@ def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
     out["mixed_48_requests"] = {
         "workload": "48 NACA 4-digit requests, 1k-20k vertices log-uniform, 70/30 Euler/SA-RANS, "
                     f"{nv} vertices in all, {steps} iterations each",
-        "precision": "Euler split, SA-RANS FP64",
+        "precision": "split (FP32 increments and dissipation, FP64 states, consistent flux and update)",
         "batched": round(vb / dt, 1), "plain_fp64_one_at_a_time": round(plain, 1),
         "speedup": round(vb / dt / plain, 2)}
     return out
@@ -239,7 +239,7 @@ def serving_leg(n: int = 32, seed: int = 11, max_iters: int = 20000) -> dict:
     """Solve latency of the serving loop (fvm2d_amd/serving.py) on a bounded
     stream: n seeded requests of 1000-4000 vertices (NACA 4-digit airfoils,
     70/30 Euler/SA-RANS), each solved to a 1e-6 residual drop, with an
-    iteration cap of max_iters (Euler in split precision, SA-RANS in FP64).
+    iteration cap of max_iters, in split precision.
     Offered all at once, the run gives the saturation rate in solves/s; then
     the same requests arrive as Poisson processes at 50 % and 90 % of that
     rate.  A latency runs from a request's arrival to its result: host
@@ -273,7 +273,7 @@ def serving_leg(n: int = 32, seed: int = 11, max_iters: int = 20000) -> dict:
     sat = run(0.0)                      # every request at t = 0: the saturation rate
     rate = sat["solves_per_s"]
     return {"workload": f"{n} requests, NACA 4-digit O-grids of 1000-4000 vertices, 70/30 "
-                        f"Euler/SA-RANS (Euler split precision, SA-RANS FP64), each to a 1e-6 "
+                        f"Euler/SA-RANS (split precision), each to a 1e-6 "
                         f"residual drop, iteration cap {max_iters}",
             "saturation": sat, "load_50": run(0.5 * rate), "load_90": run(0.9 * rate)}
 
@@ -295,7 +295,7 @@ def serving_full(n: int, seed: int, cap: int, rate: float, plain_every: int = 0,
     cfg = serving.ServeConfig(max_iters=cap, minres=1e-6, step_vertex_iters=step_vertex_iters)
     reqs = serving.poisson_arrivals(serving.workload(n, seed), rate, seed)
     out = {"workload": f"{n} requests (seed {seed}), NACA 4-digit O-grids of 1k-1M vertices, 70/30 "
-                       f"Euler/SA-RANS (Euler split precision, SA-RANS FP64), each to a 1e-6 residual "
+                       f"Euler/SA-RANS (split precision), each to a 1e-6 residual "
                        f"drop, iteration cap {cap}",
            "offered_per_s": rate, "vertices_total": int(sum(q.np for q in reqs))}
     if plain_every:

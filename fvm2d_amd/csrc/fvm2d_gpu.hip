@@ -744,7 +744,11 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     short *s_psl = s_gtl + NGTL;
 
     // 1. stage state of P, H1, H2
+    // inviscid split precision: the FP32 state is relative to the patch's
+    // first vertex (as the coordinates are), so the FP32 Green-Gauss sums
+    // cancel no free-stream pressure
     const double2 xref = F32G ? uniform_at(d.coord, v0) : make_double2(0.0, 0.0);
+    const double4 pref = F32G ? pin[v0] : make_double4(0.0, 0.0, 0.0, 0.0);
     for (int i = t; i < NL; i += kBlock) {
         const int gv = i < NP ? v0 + i : halo[i - NP];
         double4 p = pin[gv];
@@ -752,7 +756,8 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         s_prim[i] = p;
         s_coord[i] = xy;
         if (F32G) {
-            s_p32[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+            s_p32[i] = make_float4((float)(p.x - pref.x), (float)(p.y - pref.y), (float)(p.z - pref.z),
+                                   (float)(p.w - pref.w));
             s_c32[i] = make_float2((float)(xy.x - xref.x), (float)(xy.y - xref.y));
         }
         if (i < NG) s_ais[i] = d.ais[gv];
@@ -964,30 +969,44 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             else if (FLUX == 3) hllc_flux<0>(c, efr, pl, pr, f);
             else hllc_flux<1>(c, efr, pl, pr, f);
         } else {
-            // split precision: reconstruction and flux in FP32 on FP64 inputs
-            // (edge vector from FP64 coordinates, edge frame rounded from
-            // FP64); the flux returns to FP64 for the accumulation
-            V4f pl, pr, ff;
             const V4f qx1{{(float)g1.x, (float)g1.y, (float)g1.z, (float)g1.w}};
             const V4f qy1{{(float)h1.x, (float)h1.y, (float)h1.z, (float)h1.w}};
             const V4f qx2{{(float)g2.x, (float)g2.y, (float)g2.z, (float)g2.w}};
             const V4f qy2{{(float)h2.x, (float)h2.y, (float)h2.z, (float)h2.w}};
             float dxf, dyf;
-            V4f f1, f2;
             if (F32G) {
                 const float2 c1 = s_c32[e.x], c2 = s_c32[e.y];
-                const float4 a1f = s_p32[e.x], a2f = s_p32[e.y];
                 dxf = c2.x - c1.x; dyf = c2.y - c1.y;
-                f1 = V4f{{a1f.x, a1f.y, a1f.z, a1f.w}};
-                f2 = V4f{{a2f.x, a2f.y, a2f.z, a2f.w}};
             } else {
                 dxf = (float)(x2.x - x1.x); dyf = (float)(x2.y - x1.y);
-                f1 = to_f(p1); f2 = to_f(p2);
             }
+            if (FLUX == 1) {
+                // split precision, Roe: FP32 increments and dissipation on
+                // FP64 states and consistent flux (recon_incr, roe_flux_split)
+                const V4f dq{{(float)(p2.v[0] - p1.v[0]), (float)(p2.v[1] - p1.v[1]),
+                              (float)(p2.v[2] - p1.v[2]), (float)(p2.v[3] - p1.v[3])}};
+                V4f dl, dr;
+                recon_incr<LIM>(c, dxf, dyf, dq, qx1, qy1, qx2, qy2, dl, dr);
+                V4 pl, pr;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    pl.v[i] = p1.v[i] + (double)dl.v[i];
+                    pr.v[i] = p2.v[i] - (double)dr.v[i];
+                }
+                ok = roe_flux_split(c, EdgeFrame<double>{fr.x, fr.y, fr.z}, pl, pr, f);
+                if (!ok) bad = min(bad, ledge_id[k] + 1);
+                reinterpret_cast<double4 *>(s_work)[k] = make_double4(f.v[0], f.v[1], f.v[2], f.v[3]);
+                continue;
+            }
+            // split precision, KFVS and HLLC: reconstruction and flux in FP32
+            // on the FP64 states rounded (edge vector from FP64 coordinates,
+            // edge frame rounded from FP64); the flux returns to FP64 for the
+            // accumulation
+            V4f pl, pr, ff;
+            const V4f f1 = to_f(p1), f2 = to_f(p2);
             recon<LIM, float>(c, dxf, dyf, f1, f2, qx1, qy1, qx2, qy2, pl, pr);
             const EdgeFrame<float> efr{(float)fr.x, (float)fr.y, (float)fr.z};
-            if (FLUX == 1) ok = roe_flux<float>(c, efr, pl, pr, ff);
-            else if (FLUX == 2) kfvs_flux<float>(c, efr, pl, pr, ff);
+            if (FLUX == 2) kfvs_flux<float>(c, efr, pl, pr, ff);
             else if (FLUX == 3) hllc_flux<0, float>(c, efr, pl, pr, ff);
             else hllc_flux<1, float>(c, efr, pl, pr, ff);
             f = to_d(ff);

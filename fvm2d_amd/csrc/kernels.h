@@ -116,6 +116,85 @@ __device__ __forceinline__ void recon(const Consts &c, T dx, T dy, const VT<T> &
     }
 }
 
+// Split precision, MUSCL increments (src/recon.f:15-44 in increment form):
+// from the jump dq = p2 - p1 (taken in FP64, rounded) and FP32 gradients,
+// the increments dl, dr with pl = p1 + dl and pr = p2 - dr, which the caller
+// forms in FP64.  Increments are small, so FP32 keeps their relative
+// accuracy where FP32 states (a pressure near p_inf = 1/(gamma M^2), about
+// 32 at M 0.15) would not.
+template <int LIM>
+__device__ __forceinline__ void recon_incr(const Consts &c, float dx, float dy, const V4f &dq,
+                                           const V4f &qx1, const V4f &qy1, const V4f &qx2,
+                                           const V4f &qy2, V4f &dl, V4f &dr)
+{
+    const float A11 = float(c.ALBADA11), A12 = float(c.ALBADA12), A21 = float(c.ALBADA21),
+                A22 = float(c.ALBADA22);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float dqp = dq.v[i];
+        if (LIM) {
+            dl.v[i] = 0.5f * limit_albada<float>(A21 * (dx * qx1.v[i] + dy * qy1.v[i]) + A22 * dqp, dqp);
+            dr.v[i] = 0.5f * limit_albada<float>(A21 * (dx * qx2.v[i] + dy * qy2.v[i]) + A22 * dqp, dqp);
+        } else {
+            dl.v[i] = 0.5f * (A11 * (dx * qx1.v[i] + dy * qy1.v[i]) + A12 * dqp);
+            dr.v[i] = 0.5f * (A11 * (dx * qx2.v[i] + dy * qy2.v[i]) + A12 * dqp);
+        }
+    }
+}
+
+// Split precision, Roe flux (src/roe_flux.f:18-112 regrouped): the
+// consistent part Fc(L), whose momentum and energy carry the free-stream
+// pressure, in FP64 from the FP64 states; the jumps of the conserved
+// variables in FP64; the Roe averages, eigenvalues, wave strengths and the
+// upwind dissipation they make, all small or scaled by small jumps, in FP32.
+// false: a^2 <= 0 (src/roe_flux.f:60-65).
+__device__ __forceinline__ bool roe_flux_split(const Consts &c, const EdgeFrame<double> &fr, const V4 &L,
+                                               const V4 &R, V4 &f)
+{
+    const double G = c.GAMMA, G1 = c.GAMMA1;
+    const double ct = fr.ct, st = fr.st;
+    const double rl = L.v[0], ul = L.v[1], vl = L.v[2], pl = L.v[3];
+    const double rr = R.v[0], ur = R.v[1], vr = R.v[2], pr = R.v[3];
+    const double unl = ul * ct + vl * st, ql2 = ul * ul + vl * vl, qr2 = ur * ur + vr * vr;
+    // rho h = gamma/(gamma-1) p + rho q^2 / 2
+    const double Fc0 = rl * unl, Fc1 = pl * ct + rl * ul * unl, Fc2 = pl * st + rl * vl * unl;
+    const double Fc3 = (G / G1 * pl + 0.5 * rl * ql2) * unl;
+    const float du1 = (float)(rr - rl), du2 = (float)(rr * ur - rl * ul), du3 = (float)(rr * vr - rl * vl);
+    const float du4 = (float)((pr - pl) / G1 + 0.5 * (rr * qr2 - rl * ql2));
+    const float Gf = (float)G, G1f = (float)G1, h = 0.5f, ctf = (float)ct, stf = (float)st;
+    const float rlf = (float)rl, ulf = (float)ul, vlf = (float)vl, plf = (float)pl;
+    const float rrf = (float)rr, urf = (float)ur, vrf = (float)vr, prf = (float)pr;
+    const float hl = fdiv(fdiv(Gf * plf, rlf), G1f) + h * (float)ql2;
+    const float hr = fdiv(fdiv(Gf * prf, rrf), G1f) + h * (float)qr2;
+    const float rl12 = fsqrt(rlf), rr12 = fsqrt(rrf), rd = fdiv(1.0f, rl12 + rr12);
+    const float ua = (ulf * rl12 + urf * rr12) * rd;
+    const float va = (vlf * rl12 + vrf * rr12) * rd;
+    const float ha = (hl * rl12 + hr * rr12) * rd;
+    const float qa2 = ua * ua + va * va;
+    const float aa2 = G1f * (ha - h * qa2);
+    if (!(aa2 > 0.0f)) return false;
+    const float aa = fsqrt(aa2);
+    const float una = ua * ctf + va * stf;
+    const float vna = -ua * stf + va * ctf;
+    const float l1 = fminf(una - aa, 0.0f), l2 = fminf(una, 0.0f), l4 = fminf(una + aa, 0.0f);
+    const float m1 = fdiv(ctf * du2 + stf * du3 - una * du1, aa);
+    const float m2 = fdiv(G1f * (du4 - ua * du2 - va * du3 + qa2 * du1), aa * aa);
+    const float a4 = h * (m1 + m2), a1 = h * (m2 - m1), a3 = du1 - a1 - a4;
+    const float a2 = fdiv(stf * du2 - ctf * du3 + vna * du1, aa);
+    const float a1l1 = a1 * l1, a2l2 = a2 * l2, a3l3 = a3 * l2, a4l4 = a4 * l4;
+    const float aact = aa * ctf, aast = aa * stf;
+    const float Fd0 = a1l1 + a3l3 + a4l4;
+    const float Fd1 = a1l1 * (ua - aact) + a2l2 * aa * stf + a3l3 * ua + a4l4 * (ua + aact);
+    const float Fd2 = a1l1 * (va - aast) - a2l2 * aa * ctf + a3l3 * va + a4l4 * (va + aast);
+    const float Fd3 = a1l1 * (ha - una * aa) + a2l2 * aa * vna + a3l3 * h * qa2 + a4l4 * (ha + una * aa);
+    const double length = fr.len;
+    f.v[0] = (Fc0 + (double)Fd0) * length;
+    f.v[1] = (Fc1 + (double)Fd1) * length;
+    f.v[2] = (Fc2 + (double)Fd2) * length;
+    f.v[3] = (Fc3 + (double)Fd3) * length;
+    return true;
+}
+
 // Roe flux without entropy fix, src/roe_flux.f:18-112.  false: a^2 <= 0.
 template <class T = double>
 __device__ __forceinline__ bool roe_flux(const Consts &c, const EdgeFrame<T> &fr, const VT<T> &L,

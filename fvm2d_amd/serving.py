@@ -117,13 +117,11 @@ def poisson_arrivals(reqs: list[Request], rate: float, seed: int) -> list[Reques
 @dataclasses.dataclass
 class ServeConfig:
     hbm_budget: int = 64 << 30        # bytes of context state admitted at once
-    precision: int = 1                # Euler requests: 1 split precision (FP32 flux), 0 FP64
-    # SA-RANS requests: FP64.  With FP32 fluxes at low Mach on wall-resolved
-    # grids the residual norm sits at a noise floor (a residual-drop stop
-    # never triggers) and the solution leaves the FP64 one during the
-    # unsteady start (CL up to 5e-2 apart after 20 000 iterations on the
-    # 2049x513 RANS batch); Euler solves follow FP64 in split precision.
-    rans_precision: int = 0
+    precision: int = 1                # Euler requests: 1 split precision (FP32 increments and flux
+                                      # dissipation, FP64 states, consistent flux and update), 0 FP64
+    # SA-RANS requests: split precision as well; its perturbation form follows
+    # FP64 on the 2049x513 RANS batch to 20 000 iterations (CL within 2e-8)
+    rans_precision: int = 1
     minres: float = 1e-6              # residual drop that ends a solve
     max_iters: int = 2000             # iteration cap of a solve
     # work of one engine step per class (sets its chunk): long enough that the

@@ -139,13 +139,12 @@ def test_engine_timeout(model):
 
 
 def test_precision_per_flow_kind():
-    """Euler requests are served in split precision, SA-RANS in FP64 (split
-    does not follow FP64 to convergence on SA-RANS); both can be set."""
+    """Euler and SA-RANS requests are served in split precision by default;
+    each flow kind's precision can be set on its own."""
     reqs = serving.workload(40, 3)
     cfg = serving.ServeConfig()
     for q in reqs:
-        want = 0 if q.flow == "rans" else 1
-        assert serving.precision_for(q, cfg) == want
-        assert serving.inputs_for(q, cfg).precision == want
+        assert serving.precision_for(q, cfg) == 1
+        assert serving.inputs_for(q, cfg).precision == 1
     cfg2 = serving.ServeConfig(precision=0, rans_precision=1)
     assert {serving.precision_for(q, cfg2) for q in reqs} == {0, 1}
