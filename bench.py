@@ -594,14 +594,15 @@ def main() -> int:
     # are forks of this one)
     cpu = None if a.no_cpu else cpu_baseline(g, dict(kw))
     from fvm2d_amd.solver import Inputs, Solver
-    launch = {"patch_size": 0, "xcd": 1, "chosen_by": "occupancy model"}
+    launch = {"patch_size": 0, "xcd": 1, "threads": 256, "chosen_by": "occupancy model"}
     if a.tune:
         from fvm2d_amd import autotune
         from fvm2d_amd.solver import geometric
         geo0 = geometric(g.coord, g.elem, g.ptype)
         ch = autotune.tune(g, geo0, Inputs.from_kwargs(**kw), device="gfx950")
         kw["patch_size"] = ch.patch_size
-        launch = {"patch_size": ch.patch_size, "xcd": ch.xcd, "chosen_by": "autotuner",
+        kw["block_threads"] = ch.threads
+        launch = {"patch_size": ch.patch_size, "xcd": ch.xcd, "threads": ch.threads, "chosen_by": "autotuner",
                   "us_per_iter_tuning": round(ch.us_per_iter, 2)}
     s = Solver(g, Inputs.from_kwargs(**kw))
     if a.tune:

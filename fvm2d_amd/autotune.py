@@ -1,11 +1,13 @@
 """Autotuner for the launch shape of the stage kernel.
 
-The knobs: the patch size (vertices per workgroup tile, which sets the
-number of workgroups, the LDS per workgroup and the halo overhead) and the
-XCD-aware ordering of workgroups.  Candidates come from an occupancy model
-(whole rounds of resident workgroups over the device's CUs), each is timed
-on the actual mesh and flow kind over graph-replayed iterations, and the
-winner is cached per (device, mesh size, flow kind, precision).
+The knobs: the workgroup size of the stage kernel (256 threads, or 512 for
+the kinds compiled with that instance: Euler, Roe, limited), the patch size
+(vertices per workgroup tile, which sets the number of workgroups, the LDS
+per workgroup and the halo overhead) and the XCD-aware ordering of
+workgroups.  Candidates come from an occupancy model (whole rounds of
+resident workgroups over the device's CUs), each is timed on the actual mesh
+and flow kind over graph-replayed iterations, and the winner is cached per
+(device, mesh size, flow kind, precision).
 
 The choice changes no result except the last bits of the L2 residual norm
 (its partial sums follow the workgroups); states, CL and CD are the same
@@ -29,14 +31,16 @@ def default_cache_path() -> str:
 
 def candidates(np_: int, cus: int = 256, per_cu: int = 4, lo: int = 24, hi: int = 128) -> list[int]:
     """Patch sizes that fill k whole rounds of cus*per_cu workgroups for the
-    round counts around ~64 vertices a patch, plus a few fixed sizes."""
+    round counts around (lo + hi) / 2.375 vertices a patch (64 for 256
+    threads), plus a few fixed sizes in [lo, hi]."""
     slots = cus * per_cu
-    k0 = max(1, round(np_ / (64.0 * slots)))
+    mid = (lo + hi) / 2.375
+    k0 = max(1, round(np_ / (mid * slots)))
     out = set()
     for k in range(max(1, k0 - 1), k0 + 3):
         out.add(int(min(hi, max(lo, math.ceil(np_ / (k * slots))))))
-    for ps in (48, 64, 96, 128):
-        if ps * 4 <= max(np_, 4 * lo) or ps == lo:
+    for ps in (2 * lo, round(mid), 4 * lo, hi):
+        if lo <= ps <= hi and (ps * 4 <= max(np_, 4 * lo) or ps == lo):
             out.add(ps)
     return sorted(out)
 
@@ -46,7 +50,25 @@ class Choice:
     patch_size: int
     xcd: int
     us_per_iter: float
-    table: list            # (patch_size, xcd, us_per_iter) of every candidate
+    table: list            # (threads, patch_size, xcd, us_per_iter) of every candidate
+    threads: int = 256     # workgroup size of the stage kernel
+
+
+def block_sizes(inputs) -> list[int]:
+    """Workgroup sizes the stage kernel has an instance for, for this kind."""
+    wide = inputs.iflow == 1 and inputs.iflux == 1 and inputs.ilimit == 1
+    return [256, 512] if wide else [256]
+
+
+def shapes(np_: int, inputs, cus: int = 256) -> list[tuple[int, int]]:
+    """(threads, patch size) candidates: per_cu workgroups of 256 threads a
+    CU (4 inviscid, 3 viscous), half as many of 512 with patches twice as
+    large."""
+    per_cu = 3 if inputs.iflow != 1 else 4
+    out = [(256, ps) for ps in candidates(np_, cus=cus, per_cu=per_cu)]
+    if 512 in block_sizes(inputs):
+        out += [(512, ps) for ps in candidates(np_, cus=cus, per_cu=per_cu // 2, lo=48, hi=256)]
+    return out
 
 
 def key_of(dims: dict, inputs, device: str = "") -> str:
@@ -70,10 +92,12 @@ def save_cache(path: str, cache: dict) -> None:
     os.replace(tmp, path)
 
 
-def time_shape(grid, geo, inputs, patch_size: int, xcd: int, warmup: int = 8, iters: int = 48) -> float:
+def time_shape(grid, geo, inputs, patch_size: int, xcd: int, warmup: int = 8, iters: int = 48,
+               threads: int = 256) -> float:
     """Microseconds per iteration of one launch shape (graph replay)."""
     from .solver import Solver
-    inp = dataclasses.replace(inputs, patch_size=patch_size, maxiter=10**9, minres=0.0)
+    inp = dataclasses.replace(inputs, patch_size=patch_size, maxiter=10**9, minres=0.0,
+                              block_threads=threads)
     with Solver(grid, inp, geometry=geo) as s:
         s.tune(3, 64, xcd)
         s.iterate(warmup)
@@ -89,17 +113,16 @@ def tune(grid, geo, inputs, cache_path: str | None = None, device: str = "", tim
     path = cache_path or default_cache_path()
     key = key_of(geo.dims, inputs, device)
     cache = load_cache(path) if use_cache else {}
-    if key in cache:
+    if key in cache and "threads" in cache[key]:
         c = cache[key]
-        return Choice(c["patch_size"], c["xcd"], c["us_per_iter"], c["table"])
-    timer = timer or (lambda ps, x: time_shape(grid, geo, inputs, ps, x, iters=iters))
-    per_cu = 3 if inputs.iflow != 1 else 4
+        return Choice(c["patch_size"], c["xcd"], c["us_per_iter"], c["table"], c["threads"])
+    timer = timer or (lambda th, ps, x: time_shape(grid, geo, inputs, ps, x, iters=iters, threads=th))
     table = []
-    for ps in candidates(geo.dims["np"], per_cu=per_cu):
+    for th, ps in shapes(geo.dims["np"], inputs):
         for x in (1, 0):
-            table.append((ps, x, float(timer(ps, x))))
-    best = min(table, key=lambda r: r[2])
-    choice = Choice(best[0], best[1], best[2], table)
+            table.append((th, ps, x, float(timer(th, ps, x))))
+    best = min(table, key=lambda r: r[3])
+    choice = Choice(best[1], best[2], best[3], table, best[0])
     cache[key] = dataclasses.asdict(choice)
     if use_cache:
         save_cache(path, cache)

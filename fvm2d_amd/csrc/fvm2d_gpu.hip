@@ -660,7 +660,7 @@ __device__ __forceinline__ void tri_gg(const double2 *s_coord, const double4 *s_
     }
 }
 
-template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC, int WG = kBlock>
 __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevState *S,
                                            const PatchArgs &pa, const int b, const int irk,
                                            const double4 *__restrict__ pin,
@@ -749,7 +749,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     // cancel no free-stream pressure
     const double2 xref = F32G ? uniform_at(d.coord, v0) : make_double2(0.0, 0.0);
     const double4 pref = F32G ? pin[v0] : make_double4(0.0, 0.0, 0.0, 0.0);
-    for (int i = t; i < NL; i += kBlock) {
+    for (int i = t; i < NL; i += WG) {
         const int gv = i < NP ? v0 + i : halo[i - NP];
         double4 p = pin[gv];
         const double2 xy = d.coord[gv];
@@ -772,11 +772,11 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             }
         }
     }
-    for (int i = t; i <= NG; i += kBlock) s_gptr[i] = gptr[i];
-    for (int i = t; i <= NP; i += kBlock) s_pptr[i] = pptr[i];
-    for (int i = t; i < NGTL; i += kBlock) s_gtl[i] = gtl[i];
-    for (int i = t; i < NPSL; i += kBlock) s_psl[i] = pslot[i];
-    for (int i = t; i < NT; i += kBlock) {
+    for (int i = t; i <= NG; i += WG) s_gptr[i] = gptr[i];
+    for (int i = t; i <= NP; i += WG) s_pptr[i] = pptr[i];
+    for (int i = t; i < NGTL; i += WG) s_gtl[i] = gtl[i];
+    for (int i = t; i < NPSL; i += WG) s_psl[i] = pslot[i];
+    for (int i = t; i < NT; i += WG) {
         s_ltri[i] = ltri[i];
         if (VISC) s_lqinv[i] = lqinv[i];
     }
@@ -813,7 +813,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     };
     const bool bnd_early = NE * 32 >= NT * TW * 8;
     if (bnd_early)
-        for (int kb = kBlock - 1 - t; kb < NB; kb += kBlock) bnd_flux(kb);
+        for (int kb = WG - 1 - t; kb < NB; kb += WG) bnd_flux(kb);
 
     // 2. gradients of G (src/green_gauss.f): each vertex sums its triangles'
     // terms in ascending triangle order, each term recomputed from LDS.
@@ -828,7 +828,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
     if (VISC) {
         double *s_vt = reinterpret_cast<double *>(s_gx);          // 5 per triangle
-        for (int k = t; k < NTP; k += kBlock) {
+        for (int k = t; k < NTP; k += WG) {
             const short4 tr = s_ltri[k];
             V4 p1, p2, p3;
             double bx[3], by[3], dxt[4], dyt[4];
@@ -863,7 +863,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             o[4] = txy * um + tyy * vm + gpr * eyt;
         }
         __syncthreads();
-        for (int g = t; g < NP; g += kBlock) {
+        for (int g = t; g < NP; g += WG) {
             const int k1 = s_gptr[g + 1];
             for (int k = s_gptr[g]; k < k1; ++k) {
                 const int e = s_gtl[k], tri = e >> 2, m = e & 3;
@@ -888,7 +888,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         // triangles' Green-Gauss terms in ascending order, from the FP32
         // state and patch-relative coordinates (differences of nearby
         // coordinates, so the normals keep their relative accuracy)
-        for (int g = t; g < NG; g += kBlock) {
+        for (int g = t; g < NG; g += WG) {
             const float ais = (float)s_ais[g];
             float4 gx = make_float4(0.f, 0.f, 0.f, 0.f), gy = gx;
             const int k1 = s_gptr[g + 1];
@@ -917,7 +917,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             }
         }
     } else
-    for (int g = t; g < NG; g += kBlock) {
+    for (int g = t; g < NG; g += WG) {
         const double ais = s_ais[g];
         double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
         const int k1 = s_gptr[g + 1];
@@ -949,7 +949,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
 
     // 3. fluxes of the edges and boundary edges touching P
     int bad = INT_MAX;
-    for (int k = t; k < NE; k += kBlock) {
+    for (int k = t; k < NE; k += WG) {
         short2 e = ledge[k];
         const double4 fr = ledge_fr[k];
         double2 x1 = s_coord[e.x], x2 = s_coord[e.y];
@@ -1015,7 +1015,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         reinterpret_cast<double4 *>(s_work)[k] = make_double4(f.v[0], f.v[1], f.v[2], f.v[3]);
     }
     if (!bnd_early)
-        for (int kb = t; kb < NB; kb += kBlock) bnd_flux(kb);
+        for (int kb = t; kb < NB; kb += WG) bnd_flux(kb);
     if (bad != INT_MAX) {
         if (MODE == 0 && irk == 1) {
             atomicMin(&S->err1_index, bad);
@@ -1108,7 +1108,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         }
     }
     if (MODE == 1) {
-        Red rr = block_reduce<kBlock>(red);
+        Red rr = block_reduce<WG>(red);
         if (t == 0) {
             d.partial[3 * b] = rr.s;
             d.partial[3 * b + 1] = rr.m;
@@ -1117,13 +1117,21 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     }
 }
 
-template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC>
-__global__ void __launch_bounds__(kBlock, (patch_waves<VISC, TURB, PREC>()))
+// waves per SIMD an instance of the stage kernel is compiled for: NT = 512
+// puts two waves per SIMD per workgroup, two workgroups a CU
+template <bool VISC, bool TURB, int PREC, int NT, bool WIDE = false>
+constexpr int stage_waves()
+{
+    return NT != kBlock ? 4 : (WIDE ? 4 : patch_waves<VISC, TURB, PREC>());
+}
+
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC, int NT = kBlock>
+__global__ void __launch_bounds__(NT, (stage_waves<VISC, TURB, PREC, NT>()))
 k_patch(Dev d, Consts c, DevState *S, PatchArgs pa, int irk, const double4 *__restrict__ pin,
         double4 *__restrict__ pout, double4 *res_out)
 {
-    patch_body<FLUX, LIM, VISC, MODE, TURB, PREC>(d, c, S, pa, logical_block(pa.npatch, d.xcd), irk,
-                                                  pin, pout, res_out);
+    patch_body<FLUX, LIM, VISC, MODE, TURB, PREC, NT>(d, c, S, pa, logical_block(pa.npatch, d.xcd), irk,
+                                                      pin, pout, res_out);
 }
 
 // Wall distance, src/wall_dist.f:28-61: for every point the smallest
@@ -1291,8 +1299,8 @@ struct BatchMember {
 // WIDE: an SA instance compiled for 4 waves/SIMD (128 VGPRs, a few spilled),
 // launched when four workgroups fit a CU's LDS (small patches of small
 // meshes, as served solves have); the same operations, so the same bits
-template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC, bool WIDE = false>
-__global__ void __launch_bounds__(kBlock, (WIDE ? 4 : patch_waves<VISC, TURB, PREC>()))
+template <int FLUX, int LIM, bool VISC, int MODE, bool TURB, int PREC, bool WIDE = false, int NT = kBlock>
+__global__ void __launch_bounds__(NT, (stage_waves<VISC, TURB, PREC, NT, WIDE>()))
 k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work, int nwork, int xcd,
               int irk)
 {
@@ -1301,12 +1309,13 @@ k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work
     if (MODE == 0 && w.y < 0) {
         // the monitors of the member's previous iteration: they read only
         // what that iteration's last stage left, and this stage writes only
-        // its own buffer, so they run beside it
-        final_body<VISC>(M.d, M.c, M.S);
+        // its own buffer, so they run beside it (work lists hold such items
+        // only for kFinal-thread workgroups)
+        if (NT == kFinal) final_body<VISC>(M.d, M.c, M.S);
         return;
     }
-    patch_body<FLUX, LIM, VISC, MODE, TURB, PREC>(M.d, M.c, M.S, M.pa, w.y, irk, M.pbuf[irk - 1],
-                                                  M.pbuf[irk % 3], nullptr);
+    patch_body<FLUX, LIM, VISC, MODE, TURB, PREC, NT>(M.d, M.c, M.S, M.pa, w.y, irk, M.pbuf[irk - 1],
+                                                      M.pbuf[irk % 3], nullptr);
 }
 
 // vertex kernels over a work list of (member, block of kBlock vertices)
@@ -1519,6 +1528,7 @@ constexpr unsigned kCtxTag = 0x46564d43u, kBatchTag = 0x46564d42u;
 struct fvm_ctx {
     unsigned tag = kCtxTag;
     int cus = 256, per_cu = 4, lds_per_cu = 160 * 1024;   // device and occupancy
+    int threads = 256;                                      // stage kernel workgroup size
     int target = 0;                                         // patch size asked of the partition
     int max_patch = 0;
     long long halo_total = 0;
@@ -1834,9 +1844,12 @@ std::vector<int2> with_monitors(std::vector<int2> work, std::vector<int> cost, i
     return work;
 }
 
+// workgroups of the stage kernel a CU holds: per_cu waves per SIMD allow
+// per_cu workgroups of 256 threads (half as many of 512), fewer where the
+// LDS runs out
 int effective_per_cu(const fvm_ctx *x, int lds_bytes)
 {
-    return std::max(1, std::min(x->per_cu, x->lds_per_cu / (lds_bytes + 1024)));
+    return std::max(1, std::min(x->per_cu * kBlock / x->threads, x->lds_per_cu / (lds_bytes + 1024)));
 }
 
 // SA solves (Roe, limited) whose patches leave room for four workgroups a CU
@@ -1894,7 +1907,14 @@ struct StageLaunch {
     hipStream_t stream;
     int iflow, iflux, hllc_rotate, ilimit, precision;
     bool wide;                  // SA, Roe, limited: the 4-wave instance
+    int threads;                // workgroup size: 256, or 512 (Euler, Roe, limited)
 };
+
+// kinds with a 512-thread instance of the stage kernel
+bool has_wide_block(const fvm_params &q)
+{
+    return q.iflow == 1 && q.iflux == 1 && q.ilimit == 1;
+}
 
 void launch_stage(const StageLaunch &b, int irk, int mode)
 {
@@ -1902,6 +1922,17 @@ void launch_stage(const StageLaunch &b, int irk, int mode)
               [&](auto fl, auto lm, auto vi, auto tu, auto pr) {
         constexpr int F = decltype(fl)::value, L = decltype(lm)::value, P = decltype(pr)::value;
         constexpr bool V = decltype(vi)::value != 0, T = decltype(tu)::value != 0;
+        if constexpr (F == 1 && L == 1 && !V && !T) {
+            if (b.threads == 512) {
+                if (mode == 0)
+                    hipLaunchKernelGGL((k_patch_batch<F, L, V, 0, T, P, false, 512>), dim3(b.nwork), dim3(512),
+                                       b.lds, b.stream, b.mem, b.work, b.nwork, b.xcd, irk);
+                else
+                    hipLaunchKernelGGL((k_patch_batch<F, L, V, 1, T, P, false, 512>), dim3(b.nwork), dim3(512),
+                                       b.lds, b.stream, b.mem, b.work, b.nwork, b.xcd, irk);
+                return;
+            }
+        }
         if constexpr (T && F == 1 && L == 1) {
             if (b.wide) {
                 if (mode == 0)
@@ -1930,6 +1961,13 @@ void launch_residual(fvm_ctx *x, DevState *S, const double4 *pin, double4 *res)
               [&](auto fl, auto lm, auto vi, auto tu, auto pr) {
         constexpr int F = decltype(fl)::value, L = decltype(lm)::value, P = decltype(pr)::value;
         constexpr bool V = decltype(vi)::value != 0, T = decltype(tu)::value != 0;
+        if constexpr (F == 1 && L == 1 && !V && !T) {
+            if (x->threads == 512) {
+                hipLaunchKernelGGL((k_patch<F, L, V, 2, T, P, 512>), dim3(x->pa.npatch), dim3(512),
+                                   x->pa.lds_bytes, x->stream, x->d, x->c, S, x->pa, 1, pin, x->pbuf[1], res);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_patch<F, L, V, 2, T, P>), dim3(x->pa.npatch), dim3(kBlock),
                            x->pa.lds_bytes, x->stream, x->d, x->c, S, x->pa, 1, pin, x->pbuf[1], res);
     });
@@ -1941,7 +1979,7 @@ StageLaunch self_stage(const fvm_ctx *x, bool monitors = false)
     return StageLaunch{x->self_mem, monitors ? x->self_work_mon : x->self_work,
                        x->pa.npatch + (monitors ? 1 : 0), x->d.xcd, x->pa.lds_bytes, x->stream,
                        q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision,
-                       wide_sa(x, x->pa.lds_bytes)};
+                       wide_sa(x, x->pa.lds_bytes), x->threads};
 }
 
 // Captured iterations after the first may run the monitors inside the next
@@ -1950,8 +1988,7 @@ StageLaunch self_stage(const fvm_ctx *x, bool monitors = false)
 // it writes only dt, which a stopped solve no longer uses.
 bool monitors_merge(const fvm_params &q, bool vortex)
 {
-    (void)q;
-    return !vortex;
+    return !vortex && (q.block_threads == 0 || q.block_threads == kFinal);
 }
 
 BatchMember member_of(const fvm_ctx *x)
@@ -2176,6 +2213,8 @@ int reset_state(fvm_ctx *x)
 
 int check_params(const fvm_params *p)
 {
+    if (p->block_threads != 0 && p->block_threads != 256 && p->block_threads != 512) return FVM_E_INPUT;
+    if (p->block_threads == 512 && !has_wide_block(*p)) return FVM_E_INPUT;
     if (p->iflow < 1 || p->iflow > 3) return FVM_E_INPUT;
     if (p->iflux < 1 || p->iflux > 3) return FVM_E_INPUT;
     if (p->ilimit != 0 && p->ilimit != 1) return FVM_E_INPUT;
@@ -2275,14 +2314,14 @@ int build_patch_range(int np, const std::vector<int> &patch_ptr, const std::vect
                       const std::vector<int> &v2e_ptr, const std::vector<int> &v2e,
                       const std::vector<int2> &enode, const std::vector<int> &v2b_ptr,
                       const std::vector<int> &v2b, const std::vector<int2> &bnode, int iflow,
-                      bool split, int k0, int k1, PatchHost &ph)
+                      bool split, int threads, int k0, int k1, PatchHost &ph)
 {
     const int ntri = (int)tri.size(), nedge = (int)enode.size(), nbnd = (int)bnode.size();
     std::vector<int> loc(np, -1), tloc(ntri, -1), eloc(nedge, -1), bloc(nbnd, -1);
     std::vector<int> lv, tl, el, bl;
     for (int k = k0; k < k1; ++k) {
         const int v0 = patch_ptr[k], v1 = patch_ptr[k + 1], NP = v1 - v0;
-        if (NP > kBlock) return FVM_E_SIZE;
+        if (NP > threads) return FVM_E_SIZE;
         lv.clear(); tl.clear(); el.clear(); bl.clear();
         for (int v = v0; v < v1; ++v) { loc[v] = (int)lv.size(); lv.push_back(v); }
         for (int i = 0; i < NP; ++i)                       // 1-ring
@@ -2392,7 +2431,7 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
                   const std::vector<int> &v2e_ptr, const std::vector<int> &v2e,
                   const std::vector<int2> &enode, const std::vector<int> &v2b_ptr,
                   const std::vector<int> &v2b, const std::vector<int2> &bnode, int iflow,
-                  bool split, int lds_limit, PatchHost &ph)
+                  bool split, int threads, int lds_limit, PatchHost &ph)
 {
     const int npatch = (int)patch_ptr.size() - 1;
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
@@ -2402,7 +2441,7 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
     auto work = [&](int t) {
         const int k0 = (int)((long long)npatch * t / nth), k1 = (int)((long long)npatch * (t + 1) / nth);
         rcs[t] = build_patch_range(np, patch_ptr, aptr, adj, v2t_ptr, v2t, tri, v2e_ptr, v2e, enode, v2b_ptr,
-                                   v2b, bnode, iflow, split, k0, k1, part[t]);
+                                   v2b, bnode, iflow, split, threads, k0, k1, part[t]);
     };
     if (nth == 1) {
         work(0);
@@ -2452,11 +2491,11 @@ struct Layout {
 };
 
 int build_layout(const fvm_mesh *m, int iflow, int precision, int patch_size, int lds_limit,
-                 Layout &L)
+                 Layout &L, int threads = kBlock)
 {
     const int np = m->np, nt = m->nt, ne = m->ne;
     const int nsw = std::max(0, m->nswe2 - m->nswe1 + 1), nff = std::max(0, m->nffe2 - m->nffe1 + 1);
-    int target = patch_size > 0 ? std::min(patch_size, kBlock) : 128;
+    int target = patch_size > 0 ? std::min(patch_size, threads) : 128;
     // Vertices are renumbered patch by patch (compact patches, contiguous
     // ids).  Triangle and edge numbers keep the reference's order, which is
     // what fixes the summation order; only storage moves.
@@ -2547,14 +2586,14 @@ int build_layout(const fvm_mesh *m, int iflow, int precision, int patch_size, in
     for (;;) {
         L.ph = PatchHost();
         int rc = build_patches(np, part.patch_ptr, nadj_ptr, nadj, v2t_ptr, v2t, tri, v2e_ptr, v2e,
-                               enode, v2b_ptr, v2b, bnode, iflow, precision == 1, lds_limit,
+                               enode, v2b_ptr, v2b, bnode, iflow, precision == 1, threads, lds_limit,
                                L.ph);
         if (rc == FVM_OK) break;
         if (rc != FVM_E_SIZE || target <= 16) return rc;
         target /= 2;
         part = fvm::partition_vertices(np, adj_ptr, adj, target);
         // renumbering changed: rebuild everything for the smaller patches
-        return build_layout(m, iflow, precision, target, lds_limit, L);
+        return build_layout(m, iflow, precision, target, lds_limit, L, threads);
     }
     L.perm = std::move(perm); L.iperm = std::move(iperm); L.patch_ptr = std::move(part.patch_ptr);
     L.nadj_ptr = std::move(nadj_ptr); L.nadj = std::move(nadj);
@@ -2570,7 +2609,7 @@ int build_layout(const fvm_mesh *m, int iflow, int precision, int patch_size, in
 
 // Checks every index the kernels dereference against the array it indexes.
 // Returns 0 or the number of the first failed check.
-int validate_layout(const fvm_mesh *m, const Layout &L)
+int validate_layout(const fvm_mesh *m, const Layout &L, int threads = kBlock)
 {
     const int np = m->np, nt = m->nt, ne = m->ne;
     const int nb = (int)L.bnode.size();
@@ -2595,7 +2634,7 @@ int validate_layout(const fvm_mesh *m, const Layout &L)
         const int v0 = h[PH_V0], NP = h[PH_NP], NL = h[PH_NL], NG = h[PH_NG];
         const int NT = h[PH_NTRI], NE = h[PH_NEDGE], NB = h[PH_NBND];
         if (h[PH_NGTL] != ph.gptr[h[PH_GPTR] + NG] || h[PH_NPSL] != ph.pptr[h[PH_PPTR] + NP]) return 25;
-        if (v0 != L.patch_ptr[k] || NP != L.patch_ptr[k + 1] - v0 || NP < 1 || NP > kBlock) return 8;
+        if (v0 != L.patch_ptr[k] || NP != L.patch_ptr[k + 1] - v0 || NP < 1 || NP > threads) return 8;
         if (NG < NP || NL < NG || h[PH_HALO] + (NL - NP) > (int)ph.halo.size()) return 9;
         for (int i = 0; i < NL - NP; ++i) {
             int gv = ph.halo[h[PH_HALO] + i];
@@ -2777,24 +2816,27 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     x->per_cu = par->iflow == 3 ? kPatchWavesSA
                                 : (par->iflow == 2 ? kPatchWavesLaminar
                                                    : (par->precision == 1 ? kPatchWavesSplit : kPatchWavesEuler));
+    x->threads = par->block_threads ? par->block_threads : kBlock;
+    const int scale = x->threads / kBlock;          // 1, or 2 for 512-thread workgroups
     Layout L;
     if (par->patch_size > 0) {
-        rc = build_layout(m, par->iflow, par->precision, par->patch_size, lds_dev - 6 * 1024, L);
+        rc = build_layout(m, par->iflow, par->precision, par->patch_size, lds_dev - 6 * 1024, L, x->threads);
     } else {
-        // Whole rounds of resident workgroups, about 64 vertices a patch:
-        // k rounds of cus * per_cu workgroups; fewer vertices per patch
-        // while the LDS would keep fewer than per_cu workgroups on a CU.
-        const long slots = (long)cus * x->per_cu;
-        long k = std::max(1L, std::lround(np / (64.0 * slots)));
+        // Whole rounds of resident workgroups, about 64 vertices a patch
+        // (128 with 512 threads): k rounds of cus * per_cu / scale
+        // workgroups; fewer vertices per patch while the LDS would keep
+        // fewer workgroups on a CU.
+        const long wg = x->per_cu / scale, slots = (long)cus * wg;
+        long k = std::max(1L, std::lround(np / (64.0 * scale * slots)));
         for (int tries = 0;; ++tries, ++k) {
-            const int ps = (int)std::min(128L, std::max(24L, (np + k * slots - 1) / (k * slots)));
+            const int ps = (int)std::min(128L * scale, std::max(24L * scale, (np + k * slots - 1) / (k * slots)));
             L = Layout();
-            rc = build_layout(m, par->iflow, par->precision, ps, lds_dev - 6 * 1024, L);
-            if (rc || tries == 3 || ps == 24 || (L.ph.lds_max + 1024) * x->per_cu <= lds_cu) break;
+            rc = build_layout(m, par->iflow, par->precision, ps, lds_dev - 6 * 1024, L, x->threads);
+            if (rc || tries == 3 || ps == 24 * scale || (L.ph.lds_max + 1024) * wg <= lds_cu) break;
         }
     }
     if (!rc) x->lds_per_cu = lds_cu;
-    if (!rc && validate_layout(m, L)) rc = FVM_E_INPUT;
+    if (!rc && validate_layout(m, L, x->threads)) rc = FVM_E_INPUT;
     if (rc) { fvm_gpu_destroy(x); return rc; }
     x->perm = L.perm;
     const std::vector<int> &iperm = L.iperm;
@@ -3404,7 +3446,8 @@ void batch_iteration(fvm_batch *b, bool mon_prev = false, bool final = true)
         hipLaunchKernelGGL((k_tstep_batch<true, false>), dim3(b->nvwork), dim3(kBlock), 0, b->stream, b->d_mem,
                            b->d_vwork);
     StageLaunch L{b->d_mem, b->d_work, b->nwork, 1, b->lds, b->stream,
-                  q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision, wide_sa(b->mem[0], b->lds)};
+                  q.iflow, q.iflux, q.hllc_rotate, q.ilimit, q.precision, wide_sa(b->mem[0], b->lds),
+                  b->mem[0]->threads};
     StageLaunch LM = L;
     LM.work = b->d_work_mon;
     LM.nwork = b->nwork + nm;
@@ -3442,8 +3485,9 @@ int batch_graph(fvm_batch *b, int iters, hipGraphExec_t *out)
 
 bool same_kind(const fvm_params &a, const fvm_params &b)
 {
+    auto threads = [](const fvm_params &q) { return q.block_threads ? q.block_threads : kBlock; };
     return a.iflow == b.iflow && a.iflux == b.iflux && a.ilimit == b.ilimit && a.precision == b.precision &&
-           a.hllc_rotate == b.hllc_rotate && a.sa_compat == b.sa_compat;
+           a.hllc_rotate == b.hllc_rotate && a.sa_compat == b.sa_compat && threads(a) == threads(b);
 }
 
 int member_status(const DevState &h)

@@ -18,6 +18,7 @@ module fvm_gpu_mod
     real(c_double) :: mach_inf, aoa_deg, rey, cfl, minres, xref, yref
     integer(c_int) :: sa_compat, hllc_rotate, precision, patch_size
     real(c_double) :: nut_factor
+    integer(c_int) :: block_threads = 0, reserved_params = 0
   end type fvm_params
 
   ! fvm_mesh: what geometric() leaves in COMMON (src/geometric.f:15-60)

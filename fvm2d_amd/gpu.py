@@ -27,7 +27,7 @@ class Params(ctypes.Structure):
                 ("mach_inf", cd), ("aoa_deg", cd), ("rey", cd), ("cfl", cd), ("minres", cd),
                 ("xref", cd), ("yref", cd),
                 ("sa_compat", ci), ("hllc_rotate", ci), ("precision", ci), ("patch_size", ci),
-                ("nut_factor", cd)]
+                ("nut_factor", cd), ("block_threads", ci), ("reserved_params", ci)]
 
 
 class Mesh(ctypes.Structure):

@@ -46,6 +46,7 @@ class Inputs:
     nut_factor: float = 0.1
     istart: int = 1
     patch_size: int = 0       # vertices per patch of the stage kernel (0 = automatic)
+    block_threads: int = 0    # workgroup size of the stage kernel (0 = 256; 512 for Euler/Roe/limited)
 
     def params(self) -> Params:
         return Params(iflow=self.iflow, iflux=self.iflux, ilimit=self.ilimit, vortex=self.vortex,
@@ -55,7 +56,7 @@ class Inputs:
                       xref=self.xref, yref=self.yref, sa_compat=self.sa_compat,
                       hllc_rotate=self.hllc_rotate, precision=self.precision,
                       patch_size=self.patch_size,
-                      nut_factor=self.nut_factor)
+                      nut_factor=self.nut_factor, block_threads=self.block_threads)
 
     @classmethod
     def from_kwargs(cls, **kw) -> "Inputs":

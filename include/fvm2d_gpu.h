@@ -61,6 +61,10 @@ typedef struct {
                            workgroups, about 64 vertices a patch)               */
     double nut_factor;  /* initial nu_tilde = nut_factor*mu_l/rho
                            (src/initialize.f:71 uses 0.1)                       */
+    int block_threads;  /* workgroup size of the stage kernel: 0 or 256, or 512
+                           (Euler, Roe, limited only; patches up to 512
+                           vertices); the same results either way             */
+    int reserved_params; /* 0                                                   */
 } fvm_params;
 
 /* Mesh arrays as geometric() leaves them (src/geometric.f:15-60). */

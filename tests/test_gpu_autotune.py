@@ -20,13 +20,14 @@ def test_autotune_chooses_and_caches(tmp_path):
         geo = s.geo
     path = str(tmp_path / "tune.json")
     c = autotune.tune(g, geo, inp, cache_path=path, device="test", iters=16)
-    assert c.patch_size in autotune.candidates(geo.dims["np"]) and c.xcd in (0, 1)
-    assert all(t > 0 for _, _, t in c.table)
+    assert (c.threads, c.patch_size) in autotune.shapes(geo.dims["np"], inp) and c.xcd in (0, 1)
+    assert all(t > 0 for _, _, _, t in c.table) and {r[0] for r in c.table} == {256, 512}
     cache = json.load(open(path))
     assert len(cache) == 1 and list(cache.values())[0]["patch_size"] == c.patch_size
     states = []
-    for ps in (0, c.patch_size):
-        with Solver(g, Inputs.from_kwargs(patch_size=ps, **case_kwargs(name)), geometry=geo) as s:
+    for ps, th in ((0, 256), (c.patch_size, c.threads)):
+        with Solver(g, Inputs.from_kwargs(patch_size=ps, block_threads=th, **case_kwargs(name)),
+                    geometry=geo) as s:
             s.tune(3, 64, c.xcd if ps else 1)
             h, _ = s.iterate(30)
             states.append((h, s.get_state()["prim"]))

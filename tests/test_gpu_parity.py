@@ -283,3 +283,41 @@ def test_large_grid_residual_matches_oracle():
     res_o = o.residu()
     with Solver(g, Inputs.from_kwargs(**kw)) as s:
         np.testing.assert_array_equal(s.residu(prim.ravel()), res_o)
+
+
+@pytest.mark.parametrize("name,precision", [("euler_roe_transonic_65x17", 0), ("euler_roe_lim_129x33", 0),
+                                            ("euler_roe_lim_129x33", 1)])
+def test_512_thread_workgroups_same_bits(name, precision):
+    """The stage kernel's 512-thread instance (patches of up to 512 vertices)
+    adds the same terms in the same order: in FP64 states, CL and CD bit for
+    bit; fres sums per-workgroup partials, whose grouping follows the launch
+    shape.  Split precision rounds coordinates and states relative to each
+    patch's first vertex to FP32, so its last bits follow the partition:
+    states within 1e-5 of the 256-thread run's, CL and CD within 1e-6."""
+    out = []
+    for threads in (256, 512):
+        with solver_for(name, precision=precision, block_threads=threads,
+                        patch_size=0 if threads == 256 else 300) as s:
+            if threads == 512:
+                assert s.layout_info()["max_patch"] > 128
+            h, _ = s.iterate(40)
+            out.append((h, s.get_state()["prim"], s.residu(fixture(name)["prim0"])))
+    if precision == 1:
+        assert rel_err(out[1][1], out[0][1]) <= 1e-5
+        assert np.abs(out[1][0][:, 4:] - out[0][0][:, 4:]).max() <= 1e-6
+        return
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+    np.testing.assert_array_equal(out[0][0][:, [0, 2, 3, 4, 5]], out[1][0][:, [0, 2, 3, 4, 5]])
+    assert np.abs(out[0][0][:, 1] - out[1][0][:, 1]).max() <= 1e-12 * np.abs(out[0][0][:, 1]).max()
+
+
+def test_512_thread_workload_grid_matches_oracle():
+    from oracle.oracle import Oracle
+    g = mesh.naca0012_513x129()
+    kw = dict(iflow=1, iflux=1, ilimit=1, mach_inf=0.8, aoa_deg=1.25, cfl=1.0, maxiter=2000)
+    o = Oracle(g.coord, g.elem, g.ptype, **kw)
+    o.iterate(30)
+    with Solver(g, Inputs.from_kwargs(block_threads=512, **kw)) as s:
+        s.iterate(30)
+        np.testing.assert_array_equal(s.get_state()["prim"], o.get("prim"))

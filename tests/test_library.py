@@ -16,10 +16,26 @@ def test_library_exports_declared_symbols():
     assert not missing, missing
 
 
-def test_struct_layouts_match_header():
-    # fvm_params: 8 ints, 7 doubles, 4 ints, 1 double; fvm_mesh: 10 ints + 13 pointers
-    assert ctypes.sizeof(gpu.Params) == 8 * 4 + 7 * 8 + 4 * 4 + 8
-    assert ctypes.sizeof(gpu.Mesh) == 10 * 4 + 13 * 8
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes structures against the C compiler's layout of the header:
+    size and the offset of every field."""
+    import subprocess
+    fields = {"fvm_params": gpu.Params, "fvm_mesh": gpu.Mesh}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{gpu.HEADER}"', "int main(void) {"]
+    for cname, ct in fields.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        for f, _ in ct._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    (tmp_path / "layout.c").write_text("\n".join(lines) + "\n")
+    subprocess.run(["gcc", "-o", str(tmp_path / "layout"), str(tmp_path / "layout.c")], check=True)
+    got = [int(v) for v in subprocess.run([str(tmp_path / "layout")], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    want = []
+    for ct in fields.values():
+        want.append(ctypes.sizeof(ct))
+        want += [getattr(ct, f).offset for f, _ in ct._fields_]
+    assert got == want
 
 
 def test_status_strings():
@@ -45,3 +61,16 @@ def test_host_program_built():
     assert os.path.exists(build.HOST_EXE)
     r = subprocess.run([build.HOST_EXE], capture_output=True, text=True, timeout=60)
     assert "You must specify an input file." in r.stdout
+
+
+def test_512_thread_workgroups_only_for_euler_roe_limited():
+    """fvm_gpu_create refuses block_threads other than 0/256/512, and 512 for
+    kinds without that instance, before touching the device."""
+    from fvm2d_amd import mesh
+    from fvm2d_amd.gpu import FvmError
+    from fvm2d_amd.solver import Inputs, Solver
+    g = mesh.o_grid(33, 9, seed=1)
+    for kw in (dict(block_threads=384), dict(block_threads=512, iflux=2),
+               dict(block_threads=512, iflow=2, rey=100.0), dict(block_threads=512, ilimit=0)):
+        with pytest.raises(FvmError):
+            Solver(g, Inputs.from_kwargs(**kw))
