@@ -683,7 +683,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     auto hdr = [&](int k) { return uniform_at(pa.hdr, hb + k); };
     const int v0 = hdr(PH_V0), NP = hdr(PH_NP), NL = hdr(PH_NL), NG = hdr(PH_NG);
     const int NT = hdr(PH_NTRI), NE = hdr(PH_NEDGE), NB = hdr(PH_NBND);
-    const int NGTL = hdr(PH_NGTL), NPSL = hdr(PH_NPSL), NTP = VISC ? hdr(PH_NTP) : 0;
+    const int NGTL = hdr(PH_NGTL), NTP = VISC ? hdr(PH_NTP) : 0;
     extern __shared__ double4 smem4[];
     double *smem = reinterpret_cast<double *>(smem4);
     const int NLa = (NL + 1) & ~1;               // keeps every LDS array 16-byte aligned
@@ -691,16 +691,19 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     const int *halo = pa.halo + hdr(PH_HALO);
     const short4 *ltri = pa.ltri + hdr(PH_TRI);
     const double *lqinv = pa.lqinv + hdr(PH_TRI);
-    const int *gptr = pa.gptr + hdr(PH_GPTR);
-    const short *gtl = pa.gtl + hdr(PH_GTL);
     const short2 *ledge = pa.ledge + hdr(PH_EDGE);
     const int *ledge_id = pa.ledge_id + hdr(PH_EDGE);
     const double4 *ledge_fr = pa.ledge_fr + hdr(PH_EDGE);
     const short2 *lbnd = pa.lbnd + hdr(PH_BND);
     const int *lbnd_id = pa.lbnd_id + hdr(PH_BND);
-    const int *pptr = pa.pptr + hdr(PH_PPTR);
-    const short *pslot = pa.pslot + hdr(PH_PSL);
     const int t = threadIdx.x;
+    // Issue priority: a CU's workgroups are issued oldest first, so the
+    // youngest ones wait behind the others' edge fluxes.  The phases bound
+    // by load and dependency latency (gather, gradients, ordered sums and
+    // update) run at priority 1 and the edge fluxes, which have the VALU
+    // work to fill every gap, at 0 (measured on 513x129 Euler/Roe: FP64
+    // 49.6 -> 49.0 us per iteration, split precision 38.8 -> 37.4).
+    __builtin_amdgcn_s_setprio(1);
 
     // LDS carve-up for this patch
     double4 *s_prim = reinterpret_cast<double4 *>(smem);
@@ -737,13 +740,26 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     constexpr int TW = VISC ? 2 : 1;           // doubles per local triangle
     short4 *s_ltri = reinterpret_cast<short4 *>(s_work);
     double *s_lqinv = s_work + NT;
-    const int WB = max(NT * TW * 8, (NE + 2 * NB) * 32);
+    const int WB = (max(NT * TW * 8, (NE + 2 * NB) * 32) + 15) & ~15;
     int *s_gptr = reinterpret_cast<int *>(reinterpret_cast<char *>(s_work) + WB);
     int *s_pptr = s_gptr + NG + 1;
     short *s_gtl = reinterpret_cast<short *>(s_pptr + NP + 1);
     short *s_psl = s_gtl + NGTL;
 
-    // 1. stage state of P, H1, H2
+    // 1. stage state of P, H1, H2, and the patch's tables.  Every load that
+    // needs only the header is issued before any waits: the index tables
+    // (one 16-byte block a thread: gptr, pptr, gtl, pslot as laid out in
+    // LDS) and, inviscid, the local triangles (two a load; each patch's
+    // start 16-byte aligned); then the state, whose halo part waits for the
+    // halo ids.
+    const int4 *blob = pa.blob + hdr(PH_BLOB);
+    const int NBLOB = hdr(PH_NBLOB), NT2 = (NT + 1) >> 1;
+    const int4 *ltri4 = reinterpret_cast<const int4 *>(ltri);
+    int4 *s_blob = reinterpret_cast<int4 *>(s_gptr);
+    int4 *s_ltri4 = reinterpret_cast<int4 *>(s_ltri);
+    const int4 z4 = make_int4(0, 0, 0, 0);
+    const int4 bl0 = t < NBLOB ? blob[t] : z4;
+    const int4 tr0 = (!VISC && t < NT2) ? ltri4[t] : z4;
     // inviscid split precision: the FP32 state is relative to the patch's
     // first vertex (as the coordinates are), so the FP32 Green-Gauss sums
     // cancel no free-stream pressure
@@ -772,13 +788,16 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             }
         }
     }
-    for (int i = t; i <= NG; i += WG) s_gptr[i] = gptr[i];
-    for (int i = t; i <= NP; i += WG) s_pptr[i] = pptr[i];
-    for (int i = t; i < NGTL; i += WG) s_gtl[i] = gtl[i];
-    for (int i = t; i < NPSL; i += WG) s_psl[i] = pslot[i];
-    for (int i = t; i < NT; i += WG) {
-        s_ltri[i] = ltri[i];
-        if (VISC) s_lqinv[i] = lqinv[i];
+    if (t < NBLOB) s_blob[t] = bl0;
+    for (int i = t + WG; i < NBLOB; i += WG) s_blob[i] = blob[i];
+    if (VISC) {
+        for (int i = t; i < NT; i += WG) {
+            s_ltri[i] = ltri[i];
+            s_lqinv[i] = lqinv[i];
+        }
+    } else {
+        if (t < NT2) s_ltri4[t] = tr0;
+        for (int i = t + WG; i < NT2; i += WG) s_ltri4[i] = ltri4[i];
     }
     __syncthreads();
     if (stop) return;
@@ -948,6 +967,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     __syncthreads();
 
     // 3. fluxes of the edges and boundary edges touching P
+    __builtin_amdgcn_s_setprio(0);
     int bad = INT_MAX;
     for (int k = t; k < NE; k += WG) {
         short2 e = ledge[k];
@@ -1029,6 +1049,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     __syncthreads();
 
     // 4. residual of P in the reference's order, then the RK stage
+    __builtin_amdgcn_s_setprio(1);
     Red red{0.0, 0.0, 0};
     if (tl < NP) {
         const int v = v0 + tl;
@@ -1785,15 +1806,34 @@ void launch_update(fvm_ctx *x)
 }
 
 
+// Work estimate of one patch in a stage launch, for the balance of the
+// workgroups over the CUs: its edge fluxes plus its boundary edges.  A
+// patch with boundary edges evaluates them on otherwise idle threads beside
+// the gradients, but the chain of dependent loads and divisions of a
+// farfield (Steger-Warming) or wall edge lengthens the workgroup's critical
+// path far beyond its share of the instructions.  Weights in units of
+// 0.1 ns, fitted to per-CU end times measured on MI355X (s_memtime at each
+// phase boundary of every workgroup, 513x129 Euler/Roe, one round of 1017
+// workgroups): FP64 7.7 ns per edge, 116 per farfield and 70 per wall edge
+// (R^2 0.82 over 768 CU samples, against 0.16 for edges + local vertices);
+// split precision 4.6, 170 and 55 (R^2 0.87).
+int patch_work(int nedge, int nfarfield, int nsolid, int precision)
+{
+    if (precision == 1) return 46 * nedge + 1700 * nfarfield + 550 * nsolid;
+    return 77 * nedge + 1160 * nfarfield + 700 * nsolid;
+}
+
 // Order of a stage launch's work list (a performance choice only: every
 // result is the same for every order).  The XCD-aware mapping hands the
 // list to the 8 XCDs as contiguous ranges.  When a range fits one round of
 // resident workgroups, the k-th workgroup an XCD receives runs on CU
 // k mod C of it (C CUs per XCD; the dispatcher deals a round to the CUs in
-// turn), so the items are dealt heaviest first to the least loaded CU and
-// placed at that CU's positions j, j + C, ...: each CU gets a mix of heavy
-// and light patches.  With more rounds, the range is sorted heaviest first,
-// so that the last round holds the light ones.
+// turn; measured), and a CU's workgroups, issued oldest first, end when the
+// CU has done their summed work: the items are dealt heaviest first to the
+// least loaded CU, then swaps between the most loaded CU and the others
+// lower the largest load while they can; each CU's items sit at its
+// positions j, j + C, ... heaviest first.  With more rounds, the range is
+// sorted heaviest first, so that the last round holds the light ones.
 void order_work(std::vector<int2> &work, const std::vector<int> &cost, int cus, int per_cu)
 {
     const int n = (int)work.size();
@@ -1812,15 +1852,42 @@ void order_work(std::vector<int2> &work, const std::vector<int> &cost, int cus, 
         }
         // CU j owns local positions j, j + C, ... below m
         std::vector<long long> load(C, 0);
-        std::vector<int> fill(C, 0), cap(C);
+        std::vector<int> cap(C);
+        std::vector<std::vector<int>> items(C);       // indices into work
         for (int j = 0; j < C; ++j) cap[j] = j < m ? (m - 1 - j) / C + 1 : 0;
         for (int i = 0; i < m; ++i) {
             int best = -1;
             for (int j = 0; j < C; ++j)
-                if (fill[j] < cap[j] && (best < 0 || load[j] < load[best])) best = j;
-            out[lo + best + C * fill[best]] = work[idx[i]];
+                if ((int)items[j].size() < cap[j] && (best < 0 || load[j] < load[best])) best = j;
+            items[best].push_back(idx[i]);
             load[best] += cost[idx[i]];
-            fill[best]++;
+        }
+        // swaps that lower the most loaded CU's load without raising the
+        // partner's above it
+        for (int it = 0; it < 4 * m; ++it) {
+            int hi = 0;
+            for (int j = 1; j < C; ++j) if (load[j] > load[hi]) hi = j;
+            long long best_gain = 0;
+            int bj = -1, ba = -1, bb = -1;
+            for (int j = 0; j < C; ++j) {
+                if (j == hi) continue;
+                for (int a = 0; a < (int)items[hi].size(); ++a)
+                    for (int q = 0; q < (int)items[j].size(); ++q) {
+                        const long long dc = (long long)cost[items[hi][a]] - cost[items[j][q]];
+                        if (dc <= 0 || load[j] + dc >= load[hi]) continue;
+                        const long long gain = std::min(dc, load[hi] - (load[j] + dc));
+                        if (gain > best_gain) { best_gain = gain; bj = j; ba = a; bb = q; }
+                    }
+            }
+            if (bj < 0) break;
+            const long long dc = (long long)cost[items[hi][ba]] - cost[items[bj][bb]];
+            std::swap(items[hi][ba], items[bj][bb]);
+            load[hi] -= dc;
+            load[bj] += dc;
+        }
+        for (int j = 0; j < C; ++j) {
+            std::stable_sort(items[j].begin(), items[j].end(), [&](int a, int b) { return cost[a] > cost[b]; });
+            for (int f = 0; f < (int)items[j].size(); ++f) out[lo + j + C * f] = work[items[j][f]];
         }
     }
     work.swap(out);
@@ -2294,6 +2361,7 @@ int load_state(fvm_ctx *x, const double *prim_caller, const double *nut_caller)
 // ---- tables of the patch stage kernel ----
 struct PatchHost {
     std::vector<int> hdr, halo, gtri, gptr, ledge_id, lbnd_id, pptr;
+    std::vector<int4> blob;
     std::vector<short4> ltri;
     std::vector<short> gtl, pslot;
     std::vector<short2> ledge, lbnd;
@@ -2360,6 +2428,9 @@ int build_patch_range(int np, const std::vector<int> &patch_ptr, const std::vect
             if (g == NP - 1) h[PH_NTP] = (int)tl.size();     // P's triangles come first
         }
         h[PH_NTRI] = (int)tl.size();
+        // each patch's triangles start 16-byte aligned (the stage kernel
+        // copies them two at a time): an odd count gets a copy of the last
+        if (tl.size() & 1) { ph.ltri.push_back(ph.ltri.back()); ph.gtri.push_back(ph.gtri.back()); }
         // edges and boundary edges touching P, each once
         h[PH_EDGE] = (int)ph.ledge.size();
         h[PH_BND] = (int)ph.lbnd.size();
@@ -2409,6 +2480,22 @@ int build_patch_range(int np, const std::vector<int> &patch_ptr, const std::vect
         if (NL > 32767 || 4 * (int)tl.size() > 32767 || 2 * (NE + 2 * NB) > 32767) return FVM_E_SIZE;
         h[PH_NGTL] = (int)ph.gtl.size() - h[PH_GTL];
         h[PH_NPSL] = (int)ph.pslot.size() - h[PH_PSL];
+        {
+            // gptr, pptr, gtl, pslot of the patch back to back, as the stage
+            // kernel lays them out in LDS, in whole 16-byte units
+            const int nb = 4 * (NG + 1) + 4 * (NP + 1) + 2 * (h[PH_NGTL] + h[PH_NPSL]);
+            std::vector<char> img((nb + 15) & ~15, 0);
+            char *q = img.data();
+            std::memcpy(q, &ph.gptr[h[PH_GPTR]], 4 * (NG + 1)); q += 4 * (NG + 1);
+            std::memcpy(q, &ph.pptr[h[PH_PPTR]], 4 * (NP + 1)); q += 4 * (NP + 1);
+            std::memcpy(q, &ph.gtl[h[PH_GTL]], 2 * h[PH_NGTL]); q += 2 * h[PH_NGTL];
+            std::memcpy(q, &ph.pslot[h[PH_PSL]], 2 * h[PH_NPSL]);
+            h[PH_BLOB] = (int)ph.blob.size();
+            h[PH_NBLOB] = (int)img.size() / 16;
+            const size_t b0 = ph.blob.size();
+            ph.blob.resize(b0 + img.size() / 16);
+            std::memcpy(&ph.blob[b0], img.data(), img.size());
+        }
         const int lds = patch_lds_bytes(NP, NL, NG, (int)tl.size(), NE, NB, h[PH_NGTL], h[PH_NPSL],
                                         h[PH_NTP], iflow, split);
         ph.lds_max = std::max(ph.lds_max, lds);
@@ -2458,14 +2545,15 @@ int build_patches(int np, const std::vector<int> &patch_ptr, const std::vector<i
         const int off[][2] = {{PH_HALO, (int)ph.halo.size()}, {PH_TRI, (int)ph.ltri.size()},
                               {PH_GPTR, (int)ph.gptr.size()}, {PH_GTL, (int)ph.gtl.size()},
                               {PH_EDGE, (int)ph.ledge.size()}, {PH_BND, (int)ph.lbnd.size()},
-                              {PH_PPTR, (int)ph.pptr.size()}, {PH_PSL, (int)ph.pslot.size()}};
+                              {PH_PPTR, (int)ph.pptr.size()}, {PH_PSL, (int)ph.pslot.size()},
+                              {PH_BLOB, (int)ph.blob.size()}};
         for (size_t h = 0; h < q.hdr.size(); h += kPatchHdr)
             for (const auto &o : off) q.hdr[h + o[0]] += o[1];
         auto cat = [](auto &a, const auto &b) { a.insert(a.end(), b.begin(), b.end()); };
         cat(ph.hdr, q.hdr); cat(ph.halo, q.halo); cat(ph.gtri, q.gtri); cat(ph.gptr, q.gptr);
         cat(ph.ledge_id, q.ledge_id); cat(ph.lbnd_id, q.lbnd_id); cat(ph.pptr, q.pptr);
         cat(ph.ltri, q.ltri); cat(ph.gtl, q.gtl); cat(ph.pslot, q.pslot); cat(ph.ledge, q.ledge);
-        cat(ph.lbnd, q.lbnd); cat(ph.ledge_ds, q.ledge_ds);
+        cat(ph.lbnd, q.lbnd); cat(ph.ledge_ds, q.ledge_ds); cat(ph.blob, q.blob);
         ph.lds_max = std::max(ph.lds_max, q.lds_max);
         ph.sa_lds_max = std::max(ph.sa_lds_max, q.sa_lds_max);
         q = PatchHost();                          // frees the fragment
@@ -2674,6 +2762,20 @@ int validate_layout(const fvm_mesh *m, const Layout &L, int threads = kBlock)
             if (e.x < 0 || e.x >= NL || e.y < 0 || e.y >= NL || id < 0 || id >= nb) return 21;
         }
         if (h[PH_PPTR] + NP + 1 > (int)ph.pptr.size()) return 22;
+        {
+            // the blob is the LDS image of gptr, pptr, gtl, pslot
+            const int nb = 4 * (NG + 1) + 4 * (NP + 1) + 2 * (h[PH_NGTL] + h[PH_NPSL]);
+            if (h[PH_NBLOB] != (nb + 15) / 16 || h[PH_BLOB] < 0 ||
+                h[PH_BLOB] + h[PH_NBLOB] > (int)ph.blob.size() || (h[PH_TRI] & 1))
+                return 26;
+            const char *q = reinterpret_cast<const char *>(&ph.blob[h[PH_BLOB]]);
+            if (std::memcmp(q, &ph.gptr[h[PH_GPTR]], 4 * (NG + 1)) ||
+                std::memcmp(q + 4 * (NG + 1), &ph.pptr[h[PH_PPTR]], 4 * (NP + 1)) ||
+                std::memcmp(q + 4 * (NG + 1) + 4 * (NP + 1), &ph.gtl[h[PH_GTL]], 2 * h[PH_NGTL]) ||
+                std::memcmp(q + 4 * (NG + 1) + 4 * (NP + 1) + 2 * h[PH_NGTL], &ph.pslot[h[PH_PSL]],
+                            2 * h[PH_NPSL]))
+                return 27;
+        }
         const int *pp = &ph.pptr[h[PH_PPTR]];
         for (int i = 0; i < NP; ++i) {
             const int v = v0 + i;
@@ -2739,7 +2841,7 @@ long long fvm_gpu_footprint_estimate(int np, int nt, int netot, int iflow, int p
     // patch tables: local triangles (shorts, number, area), Green-Gauss and
     // residual term lists, local edges (ends, number, normal, frame), halo,
     // headers; bounded for patches down to 24 vertices
-    b += P * 800;
+    b += P * 900;
     b += 6LL * 8 * kHistCap + 64 * 1024;
     (void)iflow;
     return b;
@@ -2894,6 +2996,7 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     TRY(upload(x, (int **)&x->pa.lbnd_id, L.ph.lbnd_id));
     TRY(upload(x, (int **)&x->pa.pptr, L.ph.pptr));
     TRY(upload(x, (short **)&x->pa.pslot, L.ph.pslot));
+    TRY(upload(x, (int4 **)&x->pa.blob, L.ph.blob));
     x->pa.npatch = (int)L.patch_ptr.size() - 1;
     x->pa.lds_bytes = L.ph.lds_max;
     x->pa.sa_lds_bytes = L.ph.sa_lds_max;
@@ -2905,7 +3008,9 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
             const int *hk = &L.ph.hdr[kPatchHdr * k];
             halo += hk[PH_NL] - hk[PH_NP];
             maxp = std::max(maxp, hk[PH_NP]);
-            x->patch_cost[k] = hk[PH_NEDGE] + hk[PH_NL];
+            int nff = 0;
+            for (int q = 0; q < hk[PH_NBND]; ++q) nff += L.ph.lbnd_id[hk[PH_BND] + q] >= x->nsw;
+            x->patch_cost[k] = patch_work(hk[PH_NEDGE], nff, hk[PH_NBND] - nff, par->precision);
         }
         x->halo_total = halo; x->max_patch = maxp;
     }

@@ -27,7 +27,8 @@ constexpr int kPatchHdr = 24;
 enum {
     PH_V0 = 0, PH_NP, PH_NL, PH_NG, PH_HALO, PH_NTRI, PH_TRI, PH_GPTR, PH_GTL,
     PH_NEDGE, PH_EDGE, PH_NBND, PH_BND, PH_PPTR, PH_PSL, PH_NGTL, PH_NPSL,
-    PH_NTP                  // local triangles touching P (numbered first)
+    PH_NTP,                 // local triangles touching P (numbered first)
+    PH_BLOB, PH_NBLOB       // the patch's index tables as one 16-byte-aligned block
 };
 
 struct PatchArgs {
@@ -45,6 +46,8 @@ struct PatchArgs {
     const short2 *lbnd;     // local end points of each local boundary edge
     const int *lbnd_id;     // its index among the boundary edges (solid first)
     const int *pptr;        // per P-vertex: range in pslot (patch-relative)
+    const int4 *blob;       // per patch: gptr, pptr, gtl, pslot back to back (the
+                            // LDS image of the index tables), padded to 16 bytes
     const short *pslot;     // 2 * (term index) + (1 if subtracted)
     int npatch;
     int lds_bytes;
@@ -61,7 +64,7 @@ __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, i
     const int NLa = (NL + 1) & ~1;
     const int work = (NT * (visc ? 2 : 1) * 8) > ((NE + 2 * NB) * 32) ? NT * (visc ? 2 : 1) * 8
                                                                       : (NE + 2 * NB) * 32;
-    const int index = 4 * (NG + 1) + 4 * (NP + 1) + 2 * (NGTL + NPSL);
+    const int index = 4 * (NG + 1) + 4 * (NP + 1) + 2 * (NGTL + NPSL);   // the blob
     // ais of G, gathered with the state
     const int pre = ((NG + 1) & ~1) * 8;
     // gradients: FP64, or FP32 in the split mode (its reconstruction reads
@@ -72,7 +75,7 @@ __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, i
     // per local vertex: state and coordinates; with viscosity 2 (SA: 4)
     // doubles; inviscid split precision an FP32 state and coordinates (24 B)
     return NLa * 48 + NLa * 8 * (flow == 3 ? 4 : (visc ? 2 : (split ? 3 : 0))) + grad_region + pre +
-           work + ((index + 15) & ~15);
+           ((work + 15) & ~15) + ((index + 15) & ~15);
 }
 
 // bytes of the LDS carve-up of the SA step over one patch: state,
