@@ -757,9 +757,11 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     const int4 *ltri4 = reinterpret_cast<const int4 *>(ltri);
     int4 *s_blob = reinterpret_cast<int4 *>(s_gptr);
     int4 *s_ltri4 = reinterpret_cast<int4 *>(s_ltri);
-    const int4 z4 = make_int4(0, 0, 0, 0);
-    const int4 bl0 = t < NBLOB ? blob[t] : z4;
-    const int4 tr0 = (!VISC && t < NT2) ? ltri4[t] : z4;
+    // (clamped indices: plain global loads; a choice between the table and
+    // a zero vector compiles to a flat load from either address, the zero
+    // vector stored to scratch first: 0.6 us more per stage on 513x129)
+    const int4 bl0 = blob[min(t, NBLOB - 1)];
+    const int4 tr0 = VISC ? make_int4(0, 0, 0, 0) : ltri4[min(t, NT2 - 1)];
     // inviscid split precision: the FP32 state is relative to the patch's
     // first vertex (as the coordinates are), so the FP32 Green-Gauss sums
     // cancel no free-stream pressure
