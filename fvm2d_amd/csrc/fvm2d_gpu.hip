@@ -1393,6 +1393,7 @@ __device__ __forceinline__ void sa_patch_body(const Dev &d, const Consts &c, Dev
     const int *pptr = pa.pptr + hdr(PH_PPTR);
     const short *pslot = pa.pslot + hdr(PH_PSL);
     const int t = threadIdx.x;
+    const double nut_inf = c.nut_inf;
     extern __shared__ double4 smem4[];
     const int NGa = (NG + 1) & ~1, NTa = (NT + 1) & ~1;
     double4 *s_prim = smem4;
@@ -1473,7 +1474,10 @@ __device__ __forceinline__ void sa_patch_body(const Dev &d, const Consts &c, Dev
             const double sx = x2.y - x1.y, sy = -(x2.x - x1.x);
             const double u = 0.5 * (s_prim[nn.x].y + s_prim[nn.y].y), w = 0.5 * (s_prim[nn.x].z + s_prim[nn.y].z);
             const double un = u * sx + w * sy;
-            const double fl = (un > 0.0) ? un * s_nut[side ? nn.y : nn.x] : un * c.nut_inf;
+            // (the choice between the two values, not their addresses: an
+            // LDS and a global address make a flat load)
+            const double nin = s_nut[side ? nn.y : nn.x], nout = nut_inf;
+            const double fl = un * ((un > 0.0) ? nin : nout);
             divf = divf + 0.5 * fl;
         }
     }
