@@ -38,11 +38,12 @@ def test_serving_same_alone_and_in_the_mix():
     assert s["solves"] == len(reqs) and s["p99_latency_s"] >= s["p50_latency_s"] > 0
 
 
-@pytest.mark.skip(reason="hung once on the GPU under an earlier multi-threaded server (several "
-                         "threads making HIP calls during graph capture); the cause was not shown, "
-                         "so this case stays off the GPU; its host logic runs on the CPU in "
-                         "tests/test_serving_workload.py::test_admission_holds_the_budget")
 def test_serving_admission_holds_the_budget():
+    """Admission by footprint on the GPU: a budget of two footprints makes
+    requests wait for memory; every request still finishes.  (The server
+    is single-threaded and captures graphs in thread-local mode; the hang
+    this case once had came from an earlier server in which several threads
+    made HIP calls while another captured a graph.)"""
     reqs = serving.workload(8, 9, np_min=600, np_max=6000)
     c0 = serving.ServeConfig()
     fps = []
