@@ -710,6 +710,17 @@ def main() -> int:
                               f"stopped on a reference STOP")
     if cpu:
         line["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+        agg = cpu.get("aggregate", {}).get("value")
+        if agg:
+            line["speedup_vs_cpu_aggregate"] = round(value / agg, 1)
+        if batched:
+            # the north-star pair on one configuration: batched serving
+            # precisions against the plain FP64 path on the same requests
+            # (speedup) and against the host CPU (one core; all cores)
+            mix = batched["mixed_48_requests"]
+            mix["vs_cpu_one_core"] = round(mix["batched"] / cpu["value"], 1)
+            if agg:
+                mix["vs_cpu_aggregate"] = round(mix["batched"] / agg, 1)
     print(json.dumps(line), flush=True)
     s.close()
     return 0
