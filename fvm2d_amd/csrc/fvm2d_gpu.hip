@@ -32,6 +32,7 @@
 #include <numeric>
 #include <thread>
 #include <type_traits>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/fvm2d_gpu.h"
@@ -1547,10 +1548,30 @@ __global__ void k_gather_batch(const BatchMember *__restrict__ mem, int n, DevSt
 
 // ----------------------------------------------------------------- host side
 
-// Handles carry a tag that fvm_gpu_create / fvm_batch_create set and destroy
-// clears: a stale or foreign handle is refused (FVM_E_INPUT) instead of being
-// dereferenced.
+// Handles are registered when fvm_gpu_create / fvm_batch_create return them
+// and unregistered by destroy; every entry point looks a handle up before it
+// reads it, so a destroyed or foreign handle is refused (FVM_E_INPUT) without
+// touching freed memory.  The tag tells a context from a batch.
 constexpr unsigned kCtxTag = 0x46564d43u, kBatchTag = 0x46564d42u;
+namespace {
+std::mutex g_live_mu;
+std::unordered_set<const void *> g_live;
+void live_add(const void *h)
+{
+    std::lock_guard<std::mutex> l(g_live_mu);
+    g_live.insert(h);
+}
+void live_drop(const void *h)
+{
+    std::lock_guard<std::mutex> l(g_live_mu);
+    g_live.erase(h);
+}
+bool live(const void *h)
+{
+    std::lock_guard<std::mutex> l(g_live_mu);
+    return g_live.count(h) != 0;
+}
+}  // namespace
 
 struct fvm_ctx {
     unsigned tag = kCtxTag;
@@ -1635,11 +1656,11 @@ namespace {
         }                                                                           \
     } while (0)
 
-inline bool ctx_ok(const fvm_ctx *x) { return x && x->tag == kCtxTag; }
+inline bool ctx_ok(const fvm_ctx *x) { return x && live(x) && x->tag == kCtxTag; }
 }  // namespace
 int settle(fvm_ctx *x);
 namespace {
-inline bool batch_ok(const fvm_batch *b) { return b && b->tag == kBatchTag; }
+inline bool batch_ok(const fvm_batch *b) { return b && live(b) && b->tag == kBatchTag; }
 
 // monitor state with no error pending
 inline DevState no_errors()
@@ -3094,6 +3115,7 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     rc = load_state(x, prim.data(), nut.data());
     if (!rc) rc = configure(x, 3, 64, 1);
     if (rc) { fvm_gpu_destroy(x); return rc; }
+    live_add(x);
     *out = x;
     return FVM_OK;
 }
@@ -3101,7 +3123,7 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
 int fvm_gpu_destroy(fvm_ctx *x)
 {
     if (!x) return FVM_OK;
-    if (x->tag != kCtxTag) return FVM_E_INPUT;
+    if (!ctx_ok(x)) return FVM_E_INPUT;
     if (x->batch) (void)fvm_batch_remove(x->batch, x);
     if (x->stream) (void)hipStreamSynchronize(x->stream);
     drop_graphs(x);
@@ -3110,6 +3132,7 @@ int fvm_gpu_destroy(fvm_ctx *x)
         (void)hipStreamSynchronize(x->stream);
         (void)hipStreamDestroy(x->stream);
     }
+    live_drop(x);
     x->tag = 0;
     delete x;
     return FVM_OK;
@@ -3616,6 +3639,7 @@ int fvm_batch_create(fvm_batch **out)
     fvm_batch *b = new fvm_batch();
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return FVM_E_HIP; }
     keep_pool_memory();
+    live_add(b);
     *out = b;
     return FVM_OK;
 }
@@ -3757,6 +3781,7 @@ int fvm_batch_destroy(fvm_batch *b)
     if (b->hs) (void)hipHostFree(b->hs);
     b->hs = nullptr;
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    live_drop(b);
     b->tag = 0;
     delete b;
     return FVM_OK;
