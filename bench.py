@@ -12,9 +12,10 @@ dominant kernel (algorithmic bytes per launch / its HIP-event duration on
 the solver's stream) and the CPU baseline (the oracle, single core, on a
 bounded sample of the same workload); beside them split precision, batches
 of solves, the serving loop's p50/p99 latency on a bounded stream of
-requests solved to a 1e-6 residual drop (with converged / capped counts), and
-the viscous workloads (laminar 1025x257; SA-RANS 2049x513, a batch of 8) in
-FP64 and split precision with their own rooflines.
+requests solved to a 1e-6 residual drop (with converged / capped counts), the
+viscous workloads (laminar 1025x257; SA-RANS 2049x513, a batch of 8) in FP64
+and split precision with their own rooflines, and the same transonic workload
+with the other Riemann solvers (KFVS, HLLC as shipped, HLLC rotated).
 """
 from __future__ import annotations
 
@@ -153,9 +154,9 @@ def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
     in one launch per RK stage, FP64 and split precision; (b) 48 requests of
     the serving workload's distribution (NACA 4-digit airfoils, 1k-20k
     vertices, 70 % Euler / 30 % SA-RANS, seeded) as two batches (one per flow
-    kind, one stream each) with the serving precisions (Euler split, SA-RANS
-    FP64), against the same requests solved one after another by the plain
-    FP64 path."""
+    kind, one stream each) with the serving precisions (split precision for
+    both kinds), against the same requests solved one after another by the
+    plain FP64 path."""
     from fvm2d_amd import serving
     from fvm2d_amd.solver import Batch, Inputs, Solver
     out = {}
@@ -337,6 +338,46 @@ def serving_full(n: int, seed: int, cap: int, rate: float, plain_every: int = 0,
                 "peak_in_flight": srv.peak_in_flight, "peak_reserved_bytes": srv.peak_reserved,
                 "engine_steps": srv.steps, "engine_phase_s": {k: round(v, 2) for k, v in srv.phase.items()},
                 "errors": srv.errors[:10]})
+    return out
+
+
+def flux_legs(g, geo, kw, steps: int, warmup: int) -> dict:
+    """The transonic 513x129 workload with the other Riemann solvers, FP64,
+    `steps` iterations each after `warmup`: KFVS, HLLC as the reference ships
+    it (flux left in the edge frame; on this case it stops in its first
+    iteration on the reference's positivity check, which is reported with
+    the stage time on the free-stream state) and HLLC rotated to x-y
+    (`hllc_rotate` 1).  Each with its stage kernel's roofline (100 launches
+    between HIP events; the byte model does not depend on the flux)."""
+    from fvm2d_amd.gpu import FvmError
+    from fvm2d_amd.solver import Inputs, Solver
+    dims = geo.dims
+    npt, nt, ne = dims["np"], dims["nt"], dims["ne"]
+    B = algorithmic_bytes("k_patch", npt, nt, ne)
+    out = {}
+    for name, iflux, rot in (("kfvs", 2, 0), ("hllc", 3, 0), ("hllc_rotated", 3, 1)):
+        inp = Inputs.from_kwargs(**dict(kw, iflux=iflux, hllc_rotate=rot))
+        r = {}
+        with Solver(g, inp, geometry=geo) as x:
+            try:
+                x.iterate(warmup)
+                t0 = time.perf_counter()
+                h, _ = x.iterate(steps)
+                dt = time.perf_counter() - t0
+                r.update(vertex_iters_per_s=round(npt * h.shape[0] / dt, 1), iterations=int(h.shape[0]),
+                         cl=float(h[-1, 4]), cd=float(h[-1, 5]), fres_drop=float(h[-1, 1]))
+                t_ms = x.time_stage(100)
+            except FvmError as e:
+                r["stopped"] = {"status": e.code, "index": e.index,
+                                "iteration": int(x.monitor()[0]["iter"]) + 1}
+                t_ms = None
+        if t_ms is None:                # stage time on the free-stream state
+            with Solver(g, inp, geometry=geo) as x:
+                t_ms = x.time_stage(100)
+            r["stage_state"] = "free stream"
+        r["stage_ms"] = round(t_ms, 5)
+        r["frac"] = round(B / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        out[name] = r
     return out
 
 
@@ -523,6 +564,8 @@ def main() -> int:
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the split-precision and batched legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency leg")
+    ap.add_argument("--no-fluxes", action="store_true",
+                    help="skip the KFVS / HLLC / rotated-HLLC legs of the transonic workload")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the laminar 1025x257 and RANS 2049x513 batch-of-8 legs")
     ap.add_argument("--lam-iters", type=int, default=5000,
@@ -675,13 +718,22 @@ def main() -> int:
             t1 = time.perf_counter()
             h1, _ = s1.iterate(a.steps)
             e1 = time.perf_counter() - t1
+            t1_ms = None if a.no_kernel_timing else s1.time_stage(100)
         split = {"value": round(npt * a.steps / e1, 1), "unit": "vertex-iterations/s",
                  "dcl": abs(float(h1[-1, 4] - hist[-1, 4])), "dcd": abs(float(h1[-1, 5] - hist[-1, 5])),
                  "note": "same workload and step count with FP32 gradients, reconstruction and flux; "
                          "dcl, dcd against this FP64 run"}
+        if t1_ms:
+            # the split stage kernel moves the same HBM arrays (FP64 states;
+            # its FP32 copies live in LDS), so the same byte model
+            B1 = algorithmic_bytes("k_patch", npt, nt, ne)
+            split["roofline"] = {"kernel": "k_patch (split precision)", "kernel_ms_per_launch": round(t1_ms, 5),
+                                 "achieved": round(B1 / (t1_ms * 1e-3) / 1e9, 1), "unit": "GB/s",
+                                 "frac": round(B1 / (t1_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     batched = None if a.no_batch else batch_legs(g, s.geo, kw, value, a.warmup)
     served = None if a.no_serving else serving_leg()
     viscous = None if a.no_configs else config_legs(a.lam_iters, a.rans_iters)
+    fluxes = None if (a.no_fluxes or a.flux != 1) else flux_legs(g, s.geo, kw, a.steps, a.warmup)
 
     line = {
         "metric": "vertex-iters/s & %HBM roofline vs plain FP64; p50/p99 solve latency",
@@ -694,7 +746,7 @@ def main() -> int:
                                % {1: "Roe", 2: "KFVS", 3: "HLLC"}[a.flux],
                    "np": npt, "nt": nt, "ne": ne, "precision": "fp64", "launch": launch},
         "roofline": roofline, "cpu_baseline": cpu, "split_precision": split, "batched": batched,
-        "serving": served, "viscous_configs": viscous,
+        "serving": served, "viscous_configs": viscous, "other_fluxes": fluxes,
         "cl": float(hist[-1, 4]), "cd": float(hist[-1, 5]),
     }
     if stopped:
