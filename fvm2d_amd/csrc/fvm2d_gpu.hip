@@ -303,21 +303,37 @@ __device__ __forceinline__ Red shfl_red(Red a, int o)
     return Red{__shfl_xor(a.s, o), __shfl_xor(a.m, o), __shfl_xor(a.i, o)};
 }
 
+// (w, lane: the thread's wave and its lane in the wave)
 template <int NT>
-__device__ Red block_reduce(Red x)
+__device__ Red block_reduce(Red x, const int w, const int lane)
 {
     __shared__ double ss[NT / 64], sm[NT / 64];
     __shared__ int si[NT / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x = red_join(x, shfl_red(x, o));
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { ss[w] = x.s; sm[w] = x.m; si[w] = x.i; }
+    if (lane == 0) { ss[w] = x.s; sm[w] = x.m; si[w] = x.i; }
     __syncthreads();
     Red r{ss[0], sm[0], si[0]};
 #pragma unroll
     for (int k = 1; k < NT / 64; ++k) r = red_join(r, Red{ss[k], sm[k], si[k]});
     __syncthreads();
     return r;
+}
+
+template <int NT>
+__device__ Red block_reduce(Red x)
+{
+    return block_reduce<NT>(x, threadIdx.x >> 6, threadIdx.x & 63);
+}
+
+// The lane's position in its wave, computed where it is written (the
+// compiler can neither hoist nor share it), so that a kernel need not keep
+// its thread index live in a VGPR across a phase that needs every register.
+__device__ __forceinline__ int lane_here()
+{
+    int l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
 }
 
 // What src/clcd.f:26-34 reads at a solid point, left by the last stage for
@@ -698,6 +714,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     const short2 *lbnd = pa.lbnd + hdr(PH_BND);
     const int *lbnd_id = pa.lbnd_id + hdr(PH_BND);
     const int t = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);   // a scalar
     // Issue priority: a CU's workgroups are issued oldest first, so the
     // youngest ones wait behind the others' edge fluxes.  The phases bound
     // by load and dependency latency (gather, gradients, ordered sums and
@@ -966,7 +983,6 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         // viscosity the host recomputes them on request (same operations)
         if ((MODE == 2 || (MODE == 1 && VISC)) && g < NP) { d.qx[v0 + g] = qx; d.qy[v0 + g] = qy; }
     }
-    const int tl = t;                             // local vertex this thread updates
     __syncthreads();
 
     // 3. fluxes of the edges and boundary edges touching P
@@ -1037,8 +1053,12 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         if (!ok) bad = min(bad, ledge_id[k] + 1);
         reinterpret_cast<double4 *>(s_work)[k] = make_double4(f.v[0], f.v[1], f.v[2], f.v[3]);
     }
+    // (in the last stage the thread index is rebuilt past the edge loop from
+    // the wave and the lane: kept live across the edge fluxes it was the one
+    // VGPR that stage spilled to scratch; in the other stages, where nothing
+    // spills, rebuilding it costs more registers than it saves)
     if (!bnd_early)
-        for (int kb = t; kb < NB; kb += WG) bnd_flux(kb);
+        for (int kb = MODE == 1 ? (wave << 6) + lane_here() : t; kb < NB; kb += WG) bnd_flux(kb);
     if (bad != INT_MAX) {
         if (MODE == 0 && irk == 1) {
             atomicMin(&S->err1_index, bad);
@@ -1053,6 +1073,8 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
 
     // 4. residual of P in the reference's order, then the RK stage
     __builtin_amdgcn_s_setprio(1);
+    const int lane = MODE == 1 ? lane_here() : (t & 63);
+    const int tl = MODE == 1 ? (wave << 6) + lane : t;   // local vertex this thread updates
     Red red{0.0, 0.0, 0};
     if (tl < NP) {
         const int v = v0 + tl;
@@ -1132,8 +1154,8 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         }
     }
     if (MODE == 1) {
-        Red rr = block_reduce<WG>(red);
-        if (t == 0) {
+        Red rr = block_reduce<WG>(red, wave, lane);
+        if (tl == 0) {
             d.partial[3 * b] = rr.s;
             d.partial[3 * b + 1] = rr.m;
             d.partial[3 * b + 2] = (double)rr.i;
