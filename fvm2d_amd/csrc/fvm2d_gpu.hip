@@ -864,7 +864,6 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
     // (b) each vertex of P adds, in ascending triangle order, each triangle's
     // stresses against its own side (bx[m], by[m]) times 0.25/area: the
     // operations of the per-triangle loop of the reference, term for term.
-    double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
     if (VISC) {
         double *s_vt = reinterpret_cast<double *>(s_gx);          // 5 per triangle
         for (int k = t; k < NTP; k += WG) {
@@ -902,7 +901,11 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             o[4] = txy * um + tyy * vm + gpr * eyt;
         }
         __syncthreads();
+        // each P vertex's viscous residual waits for the update in LDS
+        // (after the index tables), not in registers through the edge fluxes
+        double *s_rv = reinterpret_cast<double *>(reinterpret_cast<char *>(s_gptr) + 16 * NBLOB);
         for (int g = t; g < NP; g += WG) {
+            double rv1 = 0.0, rv2 = 0.0, rv3 = 0.0;
             const int k1 = s_gptr[g + 1];
             for (int k = s_gptr[g]; k < k1; ++k) {
                 const int e = s_gtl[k], tri = e >> 2, m = e & 3;
@@ -919,6 +922,9 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
                 rv2 = rv2 + (o[1] * bxm + o[2] * bym) * areat;
                 rv3 = rv3 + (o[3] * bxm + o[4] * bym) * areat;
             }
+            s_rv[3 * g] = rv1;
+            s_rv[3 * g + 1] = rv2;
+            s_rv[3 * g + 2] = rv3;
         }
         __syncthreads();                 // the gradients overwrite the staged terms
     }
@@ -1084,7 +1090,12 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         if (MODE != 2) { o = d.prim[v]; dtv = d.dt[v]; cva = d.cvarea[v]; }
         V4 r;
         r.v[0] = 0.0;
-        r.v[1] = VISC ? rv1 : 0.0; r.v[2] = VISC ? rv2 : 0.0; r.v[3] = VISC ? rv3 : 0.0;
+        if (VISC) {
+            const double *s_rv = reinterpret_cast<const double *>(reinterpret_cast<const char *>(s_gptr) + 16 * NBLOB);
+            r.v[1] = s_rv[3 * tl]; r.v[2] = s_rv[3 * tl + 1]; r.v[3] = s_rv[3 * tl + 2];
+        } else {
+            r.v[1] = 0.0; r.v[2] = 0.0; r.v[3] = 0.0;
+        }
         const int k1 = s_pptr[tl + 1];
         const double4 *fl = reinterpret_cast<const double4 *>(s_work);
         auto apply = [&](const int q, const double4 f) {

@@ -74,8 +74,10 @@ __host__ __device__ inline int patch_lds_bytes(int NP, int NL, int NG, int NT, i
     const int grad_region = (((visc && NTP * 40 > grad) ? NTP * 40 : grad) + 15) & ~15;
     // per local vertex: state and coordinates; with viscosity 2 (SA: 4)
     // doubles; inviscid split precision an FP32 state and coordinates (24 B)
+    // with viscosity the viscous residual of each P vertex (3 doubles),
+    // after the blob
     return NLa * 48 + NLa * 8 * (flow == 3 ? 4 : (visc ? 2 : (split ? 3 : 0))) + grad_region + pre +
-           ((work + 15) & ~15) + ((index + 15) & ~15);
+           ((work + 15) & ~15) + ((index + 15) & ~15) + (visc ? NP * 24 : 0);
 }
 
 // bytes of the LDS carve-up of the SA step over one patch: state,
