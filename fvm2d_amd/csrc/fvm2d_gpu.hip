@@ -1007,7 +1007,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             V4 qx1{{g1.x, g1.y, g1.z, g1.w}}, qy1{{h1.x, h1.y, h1.z, h1.w}};
             V4 qx2{{g2.x, g2.y, g2.z, g2.w}}, qy2{{h2.x, h2.y, h2.z, h2.w}};
             V4 pl, pr;
-            recon<LIM>(c, x2.x - x1.x, x2.y - x1.y, p1, p2, qx1, qy1, qx2, qy2, pl, pr);
+            recon<LIM, double, FLUX == 1 && !VISC>(c, x2.x - x1.x, x2.y - x1.y, p1, p2, qx1, qy1, qx2, qy2, pl, pr);
             const EdgeFrame<double> efr{fr.x, fr.y, fr.z};
             if (FLUX == 1) ok = roe_flux(c, efr, pl, pr, f);
             else if (FLUX == 2) kfvs_flux(c, efr, pl, pr, f);

@@ -82,18 +82,25 @@ __device__ __forceinline__ EdgeFrame<T> edge_frame(T dsx, T dsy)
     return EdgeFrame<T>{fdiv(dsx, length), fdiv(dsy, length), length};
 }
 
-// van Albada limiter, src/recon.f:55-71
-template <class T>
+// van Albada limiter, src/recon.f:55-71.  SEL: the quotient is formed
+// whether or not the slopes agree in sign and then selected (the same value:
+// bot is at least 2 eps), without a branch around each of the eight
+// divisions of an edge, so the compiler can interleave their dependency
+// chains (inviscid FP64 Roe stage: 2% faster, 1.3% per iteration; the FP32
+// quotient is cheap, the KFVS instance would spill and the SA-RANS one ran
+// 1% slower, so they keep the branch).
+template <class T, bool SEL = false>
 __device__ __forceinline__ T limit_albada(T ul, T ur)
 {
-    if (ul * ur <= T(0)) return T(0);
+    if (!SEL && ul * ur <= T(0)) return T(0);
     T top = (ul * ul + T(kEps)) * ur + (ur * ur + T(kEps)) * ul;
     T bot = ul * ul + ur * ur + T(2.0 * kEps);
-    return fdiv(top, bot);
+    const T q = fdiv(top, bot);
+    return (SEL && ul * ur <= T(0)) ? T(0) : q;
 }
 
 // MUSCL states at the edge midpoint, src/recon.f:15-44
-template <int LIM, class T = double>
+template <int LIM, class T = double, bool SEL = false>
 __device__ __forceinline__ void recon(const Consts &c, T dx, T dy, const VT<T> &p1,
                                       const VT<T> &p2, const VT<T> &qx1, const VT<T> &qy1,
                                       const VT<T> &qx2, const VT<T> &qy2, VT<T> &pl, VT<T> &pr)
@@ -104,9 +111,9 @@ __device__ __forceinline__ void recon(const Consts &c, T dx, T dy, const VT<T> &
         T dqp = p2.v[i] - p1.v[i];
         if (LIM) {
             T dql = A21 * (dx * qx1.v[i] + dy * qy1.v[i]) + A22 * dqp;
-            pl.v[i] = p1.v[i] + T(0.5) * limit_albada<T>(dql, dqp);
+            pl.v[i] = p1.v[i] + T(0.5) * limit_albada<T, SEL>(dql, dqp);
             T dqr = A21 * (dx * qx2.v[i] + dy * qy2.v[i]) + A22 * dqp;
-            pr.v[i] = p2.v[i] - T(0.5) * limit_albada<T>(dqr, dqp);
+            pr.v[i] = p2.v[i] - T(0.5) * limit_albada<T, SEL>(dqr, dqp);
         } else {
             T dql = dx * qx1.v[i] + dy * qy1.v[i];
             pl.v[i] = p1.v[i] + T(0.5) * (A11 * dql + A12 * dqp);
