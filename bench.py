@@ -769,9 +769,9 @@ def main() -> int:
         if agg:
             line["speedup_vs_cpu_aggregate"] = round(value / agg, 1)
         if batched:
-            # the north-star pair on one configuration: batched serving
-            # precisions against the plain FP64 path on the same requests
-            # (speedup) and against the host CPU (one core; all cores)
+            # both ratios on one workload: the batched serving precisions
+            # against the plain FP64 path on the same requests (speedup) and
+            # against the host CPU (one core; all cores)
             mix = batched["mixed_48_requests"]
             mix["vs_cpu_one_core"] = round(mix["batched"] / cpu["value"], 1)
             if agg:
