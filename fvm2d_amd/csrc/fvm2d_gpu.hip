@@ -1971,12 +1971,16 @@ std::vector<int2> with_monitors(std::vector<int2> work, std::vector<int> cost, i
     return work;
 }
 
+// LDS a stage-kernel workgroup takes on a CU: its dynamic LDS plus the
+// kernel's 512 bytes of static LDS, in 1 KiB units
+inline int lds_alloc(int lds_bytes) { return (lds_bytes + 512 + 1023) & ~1023; }
+
 // workgroups of the stage kernel a CU holds: per_cu waves per SIMD allow
 // per_cu workgroups of 256 threads (half as many of 512), fewer where the
 // LDS runs out
 int effective_per_cu(const fvm_ctx *x, int lds_bytes)
 {
-    return std::max(1, std::min(x->per_cu * kBlock / x->threads, x->lds_per_cu / (lds_bytes + 1024)));
+    return std::max(1, std::min(x->per_cu * kBlock / x->threads, x->lds_per_cu / lds_alloc(lds_bytes)));
 }
 
 // SA solves (Roe, limited) whose patches leave room for four workgroups a CU
@@ -1984,7 +1988,7 @@ int effective_per_cu(const fvm_ctx *x, int lds_bytes)
 bool wide_sa(const fvm_ctx *x, int lds_bytes)
 {
     const fvm_params &q = x->par;
-    return q.iflow == 3 && q.iflux == 1 && q.ilimit == 1 && (lds_bytes + 1024) * 4 <= x->lds_per_cu;
+    return q.iflow == 3 && q.iflux == 1 && q.ilimit == 1 && lds_alloc(lds_bytes) * 4 <= x->lds_per_cu;
 }
 
 // workgroups a CU holds at once for the stage kernel of this solve
@@ -2994,7 +2998,7 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
             const int ps = (int)std::min(128L * scale, std::max(24L * scale, (np + k * slots - 1) / (k * slots)));
             L = Layout();
             rc = build_layout(m, par->iflow, par->precision, ps, lds_dev - 6 * 1024, L, x->threads);
-            if (rc || tries == 3 || ps == 24 * scale || (L.ph.lds_max + 1024) * wg <= lds_cu) break;
+            if (rc || tries == 3 || ps == 24 * scale || lds_alloc(L.ph.lds_max) * wg <= lds_cu) break;
         }
     }
     if (!rc) x->lds_per_cu = lds_cu;
