@@ -2734,19 +2734,15 @@ int build_layout(const fvm_mesh *m, int iflow, int precision, int patch_size, in
     for (int n = 0; n < np; ++n) orig[n] = perm[n] + 1;
 
 
-    // patch tables; halve the patch size until a patch fits the LDS budget
-    for (;;) {
-        L.ph = PatchHost();
-        int rc = build_patches(np, part.patch_ptr, nadj_ptr, nadj, v2t_ptr, v2t, tri, v2e_ptr, v2e,
-                               enode, v2b_ptr, v2b, bnode, iflow, precision == 1, threads, lds_limit,
-                               L.ph);
-        if (rc == FVM_OK) break;
-        if (rc != FVM_E_SIZE || target <= 16) return rc;
-        target /= 2;
-        part = fvm::partition_vertices(np, adj_ptr, adj, target);
-        // renumbering changed: rebuild everything for the smaller patches
-        return build_layout(m, iflow, precision, target, lds_limit, L, threads);
-    }
+    // patch tables; when a patch does not fit the LDS budget, partition and
+    // renumber again for patches of half the size
+    L.ph = PatchHost();
+    const int prc = build_patches(np, part.patch_ptr, nadj_ptr, nadj, v2t_ptr, v2t, tri, v2e_ptr, v2e,
+                                  enode, v2b_ptr, v2b, bnode, iflow, precision == 1, threads, lds_limit,
+                                  L.ph);
+    if (prc == FVM_E_SIZE && target > 16)
+        return build_layout(m, iflow, precision, target / 2, lds_limit, L, threads);
+    if (prc != FVM_OK) return prc;
     L.perm = std::move(perm); L.iperm = std::move(iperm); L.patch_ptr = std::move(part.patch_ptr);
     L.nadj_ptr = std::move(nadj_ptr); L.nadj = std::move(nadj);
     L.coord = std::move(coord); L.eds = std::move(eds); L.tri = std::move(tri);
