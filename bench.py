@@ -15,7 +15,8 @@ of solves, the serving loop's p50/p99 latency on a bounded stream of
 requests solved to a 1e-6 residual drop (with converged / capped counts), the
 viscous workloads (laminar 1025x257; SA-RANS 2049x513, a batch of 8) in FP64
 and split precision with their own rooflines, and the same transonic workload
-with the other Riemann solvers (KFVS, HLLC as shipped, HLLC rotated).
+with the other Riemann solvers (KFVS, HLLC as shipped, HLLC rotated) and on
+O-grids of 16k to 1M vertices.
 """
 from __future__ import annotations
 
@@ -341,6 +342,36 @@ def serving_full(n: int, seed: int, cap: int, rate: float, plain_every: int = 0,
     return out
 
 
+def size_legs(kw) -> list:
+    """The same Euler/Roe workload on seeded O-grids of 16 640 to 1 050 624
+    vertices (±10 % jitter), FP64 and split precision: vertex-iterations/s
+    over about half a second of iterations after as many untimed, and the
+    stage kernel's roofline (50 launches between HIP events, best of 3)."""
+    from fvm2d_amd import mesh
+    from fvm2d_amd.solver import Inputs, Solver
+    out = []
+    for ni, nj in ((257, 65), (513, 129), (1025, 257), (2049, 513)):
+        g = mesh.o_grid(ni, nj, jitter=0.1, seed=ni * 1000 + nj)
+        for prec in (0, 1):
+            with Solver(g, Inputs.from_kwargs(**dict(kw, precision=prec))) as x:
+                d = x.geo.dims
+                n = max(200, int(1e9 / d["np"]))
+                x.iterate(n)
+                t0 = time.perf_counter()
+                h, _ = x.iterate(n)
+                dt = time.perf_counter() - t0
+                t_ms = min(x.time_stage(50) for _ in range(3))
+                B = algorithmic_bytes("k_patch", d["np"], d["nt"], d["ne"])
+                info = x.layout_info()
+                out.append({"grid": f"{ni}x{nj}", "np": d["np"], "precision": "split" if prec else "fp64",
+                            "iterations": int(h.shape[0]),
+                            "vertex_iters_per_s": round(d["np"] * h.shape[0] / dt, 1),
+                            "stage_ms": round(t_ms, 5),
+                            "frac": round(B / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "patches": info["patches"], "patch_size": info["patch_size"]})
+    return out
+
+
 def flux_legs(g, geo, kw, steps: int, warmup: int) -> dict:
     """The transonic 513x129 workload with the other Riemann solvers, FP64,
     `steps` iterations each after `warmup`: KFVS, HLLC as the reference ships
@@ -567,6 +598,8 @@ def main() -> int:
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the split-precision and batched legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-latency leg")
+    ap.add_argument("--no-sizes", action="store_true",
+                    help="skip the Euler legs on O-grids of 16k to 1M vertices")
     ap.add_argument("--no-fluxes", action="store_true",
                     help="skip the KFVS / HLLC / rotated-HLLC legs of the transonic workload")
     ap.add_argument("--no-configs", action="store_true",
@@ -741,6 +774,7 @@ def main() -> int:
     served = None if a.no_serving else serving_leg()
     viscous = None if a.no_configs else config_legs(a.lam_iters, a.rans_iters)
     fluxes = None if (a.no_fluxes or a.flux != 1) else flux_legs(g, s.geo, kw, a.steps, a.warmup)
+    sizes = None if (a.no_sizes or a.flux != 1) else size_legs(kw)
 
     line = {
         "metric": "vertex-iters/s & %HBM roofline vs plain FP64; p50/p99 solve latency",
@@ -753,7 +787,7 @@ def main() -> int:
                                % {1: "Roe", 2: "KFVS", 3: "HLLC"}[a.flux],
                    "np": npt, "nt": nt, "ne": ne, "precision": "fp64", "launch": launch},
         "roofline": roofline, "cpu_baseline": cpu, "split_precision": split, "batched": batched,
-        "serving": served, "viscous_configs": viscous, "other_fluxes": fluxes,
+        "serving": served, "viscous_configs": viscous, "other_fluxes": fluxes, "by_mesh_size": sizes,
         "cl": float(hist[-1, 4]), "cd": float(hist[-1, 5]),
     }
     if stopped:
