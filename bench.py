@@ -164,7 +164,12 @@ def batch_legs(g, geo, kw, plain_value: float, warmup: int) -> dict:
     steps = 200
     alphas = [2.0 * k for k in range(8)]
     for prec, name in ((0, "fp64"), (1, "split")):
-        sol = [Solver(g, Inputs.from_kwargs(**dict(kw, aoa_deg=al, precision=prec)), geometry=geo)
+        # 256-thread workgroups: with eight solves in a launch the smaller
+        # patches balance better (8 x 513x129: 1.99e9 against 1.95e9 FP64,
+        # 2.46e9 against 2.37e9 split with the 512 threads a single solve of
+        # this size gets)
+        sol = [Solver(g, Inputs.from_kwargs(**dict(kw, aoa_deg=al, precision=prec, block_threads=256)),
+                      geometry=geo)
                for al in alphas]
         with Batch() as b:
             for x in sol:
@@ -368,7 +373,8 @@ def size_legs(kw) -> list:
                             "vertex_iters_per_s": round(d["np"] * h.shape[0] / dt, 1),
                             "stage_ms": round(t_ms, 5),
                             "frac": round(B / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                            "patches": info["patches"], "patch_size": info["patch_size"]})
+                            "patches": info["patches"], "patch_size": info["patch_size"],
+                            "threads": info["threads"]})
     return out
 
 
@@ -673,7 +679,7 @@ def main() -> int:
     # are forks of this one)
     cpu = None if a.no_cpu else cpu_baseline(g, dict(kw))
     from fvm2d_amd.solver import Inputs, Solver
-    launch = {"patch_size": 0, "xcd": 1, "threads": 256, "chosen_by": "occupancy model"}
+    launch = {"patch_size": 0, "xcd": 1, "chosen_by": "automatic launch shape (occupancy model)"}
     if a.tune:
         from fvm2d_amd import autotune
         from fvm2d_amd.solver import geometric
@@ -689,7 +695,7 @@ def main() -> int:
     if a.variant is not None:
         s.tune(a.variant, a.vb, 1)
     info = s.layout_info()
-    launch.update(patches=info["patches"], patch_size=info["patch_size"],
+    launch.update(patches=info["patches"], patch_size=info["patch_size"], threads=info["threads"],
                   lds_bytes=info["lds_bytes"], workgroups_per_cu=info["workgroups_per_cu"])
     dims = s.geo.dims
     npt, nt, ne = dims["np"], dims["nt"], dims["ne"]

@@ -533,9 +533,11 @@ __device__ __forceinline__ void wall_force(double4 w1, double4 w2, double2 sn, d
     }
 }
 
-// (sum, max, argmax) and a force pair over the workgroup in one pass of
-// barriers; each component follows the butterfly and the ordered sum of
-// wavefront totals of block_reduce
+// (sum, max, argmax) and a force pair over the first NT threads of the
+// workgroup in one pass of barriers; each component follows the butterfly
+// and the ordered sum of wavefront totals of block_reduce.  Every thread of
+// the workgroup calls it (a larger workgroup's other waves only meet the
+// barrier), so the result is the same bits in a workgroup of any size.
 template <int NT>
 __device__ void block_reduce_final(Red &x, double2 &f)
 {
@@ -549,7 +551,7 @@ __device__ void block_reduce_final(Red &x, double2 &f)
         f.y = f.y + __shfl_xor(f.y, o);
     }
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { ss[w] = x.s; sm[w] = x.m; si[w] = x.i; s2[w] = f; }
+    if ((threadIdx.x & 63) == 0 && w < NT / 64) { ss[w] = x.s; sm[w] = x.m; si[w] = x.i; s2[w] = f; }
     __syncthreads();
     Red r{ss[0], sm[0], si[0]};
     double2 g = s2[0];
@@ -562,7 +564,7 @@ __device__ void block_reduce_final(Red &x, double2 &f)
     f = g;
 }
 
-template <bool VISC>
+template <bool VISC, int NT = kFinal>
 __device__ __forceinline__ void final_body(const Dev &d, const Consts &c, DevState *S)
 {
     // Every load the monitors need (the solve's state words, the first
@@ -572,11 +574,13 @@ __device__ __forceinline__ void final_body(const Dev &d, const Consts &c, DevSta
     const int done = S->done, err0 = S->err_code, errv = S->errv_index;
     const int iter = S->iter, nhist = S->nhist;
     const double fres1 = S->fres1;
+    // the first kFinal threads do the work in a workgroup of NT threads
     const int t = threadIdx.x;
+    const bool act = NT == kFinal || t < kFinal;
     Red acc{0.0, 0.0, 0};
     double xf = 0.0, yf = 0.0;
     {
-        const bool hp = t < d.nparts, hw = t < d.nsw;
+        const bool hp = act && t < d.nparts, hw = act && t < d.nsw;
         double ps = 0.0, pm = 0.0, pi = 0.0;
         double4 w1 = make_double4(0.0, 0.0, 0.0, 0.0), w2 = w1;
         double2 sn = make_double2(0.0, 0.0);
@@ -586,10 +590,10 @@ __device__ __forceinline__ void final_body(const Dev &d, const Consts &c, DevSta
         if (hw) wall_force<VISC>(w1, w2, sn, xf, yf);
     }
 #pragma unroll 1
-    for (int k = t + kFinal; k < d.nparts; k += kFinal)
+    for (int k = t + kFinal; act && k < d.nparts; k += kFinal)
         acc = red_join(acc, Red{d.partial[3 * k], d.partial[3 * k + 1], (int)d.partial[3 * k + 2]});
 #pragma unroll 1
-    for (int b = t + kFinal; b < d.nsw; b += kFinal)
+    for (int b = t + kFinal; act && b < d.nsw; b += kFinal)
         wall_force<VISC>(d.wall[2 * b], d.wall[2 * b + 1], d.wall_s[b], xf, yf);
     double2 f = make_double2(xf, yf);
     block_reduce_final<kFinal>(acc, f);
@@ -1367,8 +1371,8 @@ k_patch_batch(const BatchMember *__restrict__ mem, const int2 *__restrict__ work
         // the monitors of the member's previous iteration: they read only
         // what that iteration's last stage left, and this stage writes only
         // its own buffer, so they run beside it (work lists hold such items
-        // only for kFinal-thread workgroups)
-        if (NT == kFinal) final_body<VISC>(M.d, M.c, M.S);
+        // for workgroups of 256 or 512 threads)
+        final_body<VISC, NT>(M.d, M.c, M.S);
         return;
     }
     patch_body<FLUX, LIM, VISC, MODE, TURB, PREC, NT>(M.d, M.c, M.S, M.pa, w.y, irk, M.pbuf[irk - 1],
@@ -2119,7 +2123,8 @@ StageLaunch self_stage(const fvm_ctx *x, bool monitors = false)
 // it writes only dt, which a stopped solve no longer uses.
 bool monitors_merge(const fvm_params &q, bool vortex)
 {
-    return !vortex && (q.block_threads == 0 || q.block_threads == kFinal);
+    (void)q;
+    return !vortex;
 }
 
 BatchMember member_of(const fvm_ctx *x)
@@ -2947,6 +2952,8 @@ int fvm_gpu_layout_info(const fvm_ctx *x, long long *info)
 
 long long fvm_gpu_footprint(const fvm_ctx *x) { return ctx_ok(x) ? x->bytes : 0; }
 
+int fvm_gpu_workgroup_threads(const fvm_ctx *x) { return ctx_ok(x) ? x->threads : FVM_E_INPUT; }
+
 int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
 {
     *out = nullptr;
@@ -2978,7 +2985,13 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     x->per_cu = par->iflow == 3 ? kPatchWavesSA
                                 : (par->iflow == 2 ? kPatchWavesLaminar
                                                    : (par->precision == 1 ? kPatchWavesSplit : kPatchWavesEuler));
-    x->threads = par->block_threads ? par->block_threads : kBlock;
+    // Workgroup size: the caller's, or with the automatic launch shape 512
+    // threads for Euler/Roe/limited solves of at least 32 768 vertices (two
+    // workgroups a CU on patches of about 130 vertices, fewer halo vertices
+    // per patch vertex: 513x129 FP64 41.8 against 45.0 us per iteration),
+    // else 256.
+    x->threads = par->block_threads ? par->block_threads
+                                    : ((par->patch_size == 0 && has_wide_block(*par) && np >= 32768) ? 512 : kBlock);
     const int scale = x->threads / kBlock;          // 1, or 2 for 512-thread workgroups
     Layout L;
     if (par->patch_size > 0) {
@@ -2986,15 +2999,29 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
     } else {
         // Whole rounds of resident workgroups, about 64 vertices a patch
         // (128 with 512 threads): k rounds of cus * per_cu / scale
-        // workgroups; fewer vertices per patch while the LDS would keep
-        // fewer workgroups on a CU.
+        // workgroups less one, the slot where the first stage runs the
+        // previous iteration's monitors beside the patches (with every slot
+        // taken they would wait for a second round); fewer vertices per patch
+        // while the LDS would keep fewer workgroups on a CU.
         const long wg = x->per_cu / scale, slots = (long)cus * wg;
         long k = std::max(1L, std::lround(np / (64.0 * scale * slots)));
-        for (int tries = 0;; ++tries, ++k) {
-            const int ps = (int)std::min(128L * scale, std::max(24L * scale, (np + k * slots - 1) / (k * slots)));
+        bool done = false;
+        if (scale == 2 && np / (192L * slots) >= 4) {
+            // large meshes: patches of about 190 vertices where two still
+            // fit a CU's LDS (1 050 624 vertices, FP64: 490 against 590 us
+            // per iteration at 129)
+            const long room = std::lround(np / (192.0 * slots)) * slots - 1;
+            L = Layout();
+            rc = build_layout(m, par->iflow, par->precision, (int)((np + room - 1) / room), lds_dev - 6 * 1024,
+                              L, x->threads);
+            done = rc != FVM_OK || lds_alloc(L.ph.lds_max) * wg <= lds_cu;
+        }
+        for (int tries = 0; !done; ++tries, ++k) {
+            const long room = k * slots - 1;
+            const int ps = (int)std::min(128L * scale, std::max(24L * scale, (np + room - 1) / room));
             L = Layout();
             rc = build_layout(m, par->iflow, par->precision, ps, lds_dev - 6 * 1024, L, x->threads);
-            if (rc || tries == 3 || ps == 24 * scale || lds_alloc(L.ph.lds_max) * wg <= lds_cu) break;
+            if (rc || tries == 3 || ps == 24 * scale || lds_alloc(L.ph.lds_max) * wg <= lds_cu) done = true;
         }
     }
     if (!rc) x->lds_per_cu = lds_cu;
@@ -3650,11 +3677,12 @@ int batch_graph(fvm_batch *b, int iters, hipGraphExec_t *out)
     return end_capture(b->stream, out);
 }
 
-bool same_kind(const fvm_params &a, const fvm_params &b)
+// members share the kernel instances: the same kind of solve and workgroup
+bool same_kind(const fvm_ctx *x, const fvm_ctx *y)
 {
-    auto threads = [](const fvm_params &q) { return q.block_threads ? q.block_threads : kBlock; };
+    const fvm_params &a = x->par, &b = y->par;
     return a.iflow == b.iflow && a.iflux == b.iflux && a.ilimit == b.ilimit && a.precision == b.precision &&
-           a.hllc_rotate == b.hllc_rotate && a.sa_compat == b.sa_compat && threads(a) == threads(b);
+           a.hllc_rotate == b.hllc_rotate && a.sa_compat == b.sa_compat && x->threads == y->threads;
 }
 
 int member_status(const DevState &h)
@@ -3704,7 +3732,7 @@ namespace {
 int fvm_batch_add(fvm_batch *b, fvm_ctx *x)
 {
     if (!batch_ok(b) || !ctx_ok(x) || x->batch || x->variant != 3) return FVM_E_INPUT;
-    if (!b->mem.empty() && !same_kind(b->mem[0]->par, x->par)) return FVM_E_INPUT;
+    if (!b->mem.empty() && !same_kind(b->mem[0], x)) return FVM_E_INPUT;
     int rc = batch_wait(b);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(x->stream));

@@ -115,6 +115,11 @@ module fvm_gpu_mod
       integer(c_long_long), intent(out) :: info(*)
     end function
 
+    integer(c_int) function fvm_gpu_workgroup_threads(ctx) bind(C, name="fvm_gpu_workgroup_threads")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
     integer(c_long_long) function fvm_gpu_footprint(ctx) bind(C, name="fvm_gpu_footprint")
       import :: c_long_long, c_ptr
       type(c_ptr), value :: ctx

@@ -93,6 +93,8 @@ def lib():
     L.fvm_patch_plan.restype = ci
     L.fvm_patch_plan.argtypes = [ctypes.POINTER(Mesh), ci, ci, ci, ci, vp]
     L.fvm_gpu_layout_info.argtypes = [vp, vp]
+    L.fvm_gpu_workgroup_threads.restype = ci
+    L.fvm_gpu_workgroup_threads.argtypes = [vp]
     L.fvm_gpu_monitor.restype = ci
     L.fvm_gpu_monitor.argtypes = [vp, vp]
     L.fvm_batch_create.restype = ci

@@ -306,7 +306,9 @@ class Solver:
         self.L.fvm_gpu_layout_info(self.h, _p(a))
         keys = ("patches", "max_patch", "lds_bytes", "workgroups_per_cu", "cus", "patch_size",
                 "halo_total", "variant")
-        return {k: int(v) for k, v in zip(keys, a)}
+        out = {k: int(v) for k, v in zip(keys, a)}
+        out["threads"] = int(self.L.fvm_gpu_workgroup_threads(self.h))
+        return out
 
 
 class Batch:

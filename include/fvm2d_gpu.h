@@ -61,9 +61,12 @@ typedef struct {
                            workgroups, about 64 vertices a patch)               */
     double nut_factor;  /* initial nu_tilde = nut_factor*mu_l/rho
                            (src/initialize.f:71 uses 0.1)                       */
-    int block_threads;  /* workgroup size of the stage kernel: 0 or 256, or 512
+    int block_threads;  /* workgroup size of the stage kernel: 256, or 512
                            (Euler, Roe, limited only; patches up to 512
-                           vertices); the same results either way             */
+                           vertices); 0 = automatic: 512 for Euler, Roe,
+                           limited solves of at least 32768 vertices with the
+                           automatic patch size, else 256.  The same results
+                           either way; batch members share one size          */
     int reserved_params; /* 0                                                   */
 } fvm_params;
 
@@ -236,6 +239,12 @@ int fvm_gpu_monitor(const fvm_ctx *ctx, double *mon);
  * vertices over all patches, kernel variant.  No counterpart in the
  * reference (its loops have no launch shape); for tuning and reports. */
 int fvm_gpu_layout_info(const fvm_ctx *ctx, long long *info);
+
+/* Threads per workgroup of the context's stage kernel (256 or 512: the
+ * caller's block_threads, or the automatic choice when it is 0), or
+ * FVM_E_INPUT for a handle that is not a live context.  No counterpart in
+ * the reference; for tuning and reports. */
+int fvm_gpu_workgroup_threads(const fvm_ctx *ctx);
 
 long long fvm_gpu_footprint(const fvm_ctx *ctx);
 /* Predicted footprint before creating a context. */

@@ -100,6 +100,20 @@ def test_batch_rejects_other_kind():
             b.add(a)                                   # already a member
 
 
+def test_batch_members_share_the_workgroup_size():
+    """Members run the same kernel instances: a 512-thread context does not
+    join a batch of 256-thread ones, and the workgroup size the library
+    reports is the one asked for, or its automatic choice (256 below 32768
+    vertices)."""
+    g, kw = euler_members()[0]
+    with Solver(g, Inputs.from_kwargs(**dict(EULER, **kw))) as a, \
+            Solver(g, Inputs.from_kwargs(**dict(EULER, block_threads=512, **kw))) as w, Batch() as b:
+        assert a.layout_info()["threads"] == 256 and w.layout_info()["threads"] == 512
+        b.add(a)
+        with pytest.raises(FvmError):
+            b.add(w)
+
+
 @pytest.mark.parametrize("name,precision", [("rans_roe_97x25", 0), ("laminar_roe_97x25", 1)])
 def test_batch_viscous_same_as_alone(name, precision):
     """SA (time step, SA step and copy as batched vertex kernels) and the
