@@ -1,7 +1,8 @@
 """Autotuner for the launch shape of the stage kernel.
 
 The knobs: the workgroup size of the stage kernel (256 threads, or 512 for
-the kinds compiled with that instance: Euler, Roe, limited), the patch size
+the kinds compiled with that instance: Euler with the limiter, any flux), the
+patch size
 (vertices per workgroup tile, which sets the number of workgroups, the LDS
 per workgroup and the halo overhead) and the XCD-aware ordering of
 workgroups.  Candidates come from an occupancy model (whole rounds of
@@ -56,7 +57,7 @@ class Choice:
 
 def block_sizes(inputs) -> list[int]:
     """Workgroup sizes the stage kernel has an instance for, for this kind."""
-    wide = inputs.iflow == 1 and inputs.iflux == 1 and inputs.ilimit == 1
+    wide = inputs.iflow == 1 and inputs.ilimit == 1
     return [256, 512] if wide else [256]
 
 

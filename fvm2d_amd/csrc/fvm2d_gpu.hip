@@ -340,9 +340,13 @@ __device__ __forceinline__ int lane_here()
 // every solid edge the point ends: the new pressure and, with viscosity, the
 // stress of the last residual's gradients.  k_final then adds the edges in
 // the reference's order without gathering anything else.
+// [k0, k1) is the point's range of boundary-edge ends (v2b_ptr[v],
+// v2b_ptr[v + 1]), which the caller loads early; the entries are read two at
+// a time before either is written, so a point on a closed wall (two ends)
+// waits for one load, not two in a row.
 template <bool VISC>
-__device__ __forceinline__ void wall_terms(const Dev &d, int v, double p, double mu, double4 qx,
-                                           double4 qy)
+__device__ __forceinline__ void wall_terms(const Dev &d, int k0, int k1, double p, double mu,
+                                           double4 qx, double4 qy)
 {
     double4 w = make_double4(p, 0.0, 0.0, 0.0);
     if (VISC) {
@@ -350,7 +354,13 @@ __device__ __forceinline__ void wall_terms(const Dev &d, int v, double p, double
         w.z = mu * (qy.y + qx.z);
         w.w = 2.0 / 3.0 * mu * (2.0 * qy.z - qx.y);
     }
-    for (int k = d.v2b_ptr[v]; k < d.v2b_ptr[v + 1]; ++k) {
+    int k = k0;
+    for (; k + 1 < k1; k += 2) {
+        const int qa = d.v2b[k], qb = d.v2b[k + 1];
+        if ((qa >> 1) < d.nsw) d.wall[qa] = w;
+        if ((qb >> 1) < d.nsw) d.wall[qb] = w;
+    }
+    if (k < k1) {
         const int q = d.v2b[k];
         if ((q >> 1) < d.nsw) d.wall[q] = w;
     }
@@ -382,7 +392,9 @@ __global__ void __launch_bounds__(kBlock) k_update(Dev d, Consts c, DevState *S)
         q.w = c.GAMMA1 * (un.v[3] - 0.5 * (sq(un.v[1]) + sq(un.v[2])) / un.v[0]);
         d.prim[v] = q;
         if (LAST) {
-            if (d.solid[v]) wall_terms<VISC>(d, v, q.w, VISC ? d.mu[v] : 0.0, d.qx[v], d.qy[v]);
+            if (d.solid[v])
+                wall_terms<VISC>(d, d.v2b_ptr[v], d.v2b_ptr[v + 1], q.w, VISC ? d.mu[v] : 0.0, d.qx[v],
+                                 d.qy[v]);
             double gra = q.x - po.v[0];
             double dif = fabs(gra);
             red.s = sq(gra);
@@ -1092,6 +1104,19 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         double4 o = make_double4(0.0, 0.0, 0.0, 0.0);
         double dtv = 0.0, cva = 1.0;
         if (MODE != 2) { o = d.prim[v]; dtv = d.dt[v]; cva = d.cvarea[v]; }
+        // and the last stage's per-point inputs: issued here, they arrive
+        // during the sum; after the state's store the compiler may not hoist
+        // them (the arrays may alias it), and they cost a round trip each
+        double drv = 0.0;
+        int orv = 0, wk0 = 0, wk1 = 0;
+        bool sol = false;
+        if (MODE == 1) {
+            if (!TURB) drv = d.drmin[v];
+            orv = d.orig[v];
+            sol = d.solid[v] != 0;
+            wk0 = d.v2b_ptr[v];
+            wk1 = d.v2b_ptr[v + 1];
+        }
         V4 r;
         r.v[0] = 0.0;
         if (VISC) {
@@ -1135,7 +1160,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             const double f = dtv / cva;
 #pragma unroll
             for (int j = 0; j < 4; ++j) un.v[j] = a * u0.v[j] + bb * (u1.v[j] - f * r.v[j]);
-            if (VISC && d.solid[v]) { un.v[1] = 0.0; un.v[2] = 0.0; }
+            if (VISC && (MODE == 1 ? sol : d.solid[v] != 0)) { un.v[1] = 0.0; un.v[2] = 0.0; }
             double4 q;
             q.x = un.v[0];
             q.y = un.v[1] / un.v[0];
@@ -1143,21 +1168,21 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
             q.w = c.GAMMA1 * (un.v[3] - 0.5 * (sq(un.v[1]) + sq(un.v[2])) / un.v[0]);
             pout[v] = q;
             if (MODE == 1) {
-                if (d.solid[v])
-                    wall_terms<VISC>(d, v, q.w, VISC ? s_mu[tl] : 0.0, d.qx[v], d.qy[v]);
+                if (sol)
+                    wall_terms<VISC>(d, wk0, wk1, q.w, VISC ? s_mu[tl] : 0.0, d.qx[v], d.qy[v]);
                 const double gra = q.x - o.x;
                 const double dif = fabs(gra);
                 red.s = sq(gra);
-                if (dif > 0.0) { red.m = dif; red.i = d.orig[v]; }
+                if (dif > 0.0) { red.m = dif; red.i = orv; }
                 if (!TURB) {
                     if (q.x <= 0.0 || q.w <= 0.0) {
-                        atomicMin(&S->errv_index, d.orig[v]);
+                        atomicMin(&S->errv_index, orv);
                         S->err_stage = irk;
                     }
                     // time_step of the next iteration (src/time_step.f:13-28)
                     double qq = sqrt(sq(q.y) + sq(q.z));
                     double aa = sqrt(c.GAMMA * q.w / q.x);
-                    double dr = d.drmin[v];
+                    double dr = drv;
                     if (VISC) {
                         double dtv = 2.0 * c.GAMMA * s_mu[tl] / (q.x * c.prandtl);
                         d.dt[v] = c.cfl * sq(dr) / (dr * (qq + aa) + dtv);
@@ -2042,13 +2067,15 @@ struct StageLaunch {
     hipStream_t stream;
     int iflow, iflux, hllc_rotate, ilimit, precision;
     bool wide;                  // SA, Roe, limited: the 4-wave instance
-    int threads;                // workgroup size: 256, or 512 (Euler, Roe, limited)
+    int threads;                // workgroup size: 256, or 512 (Euler, limited)
 };
 
-// kinds with a 512-thread instance of the stage kernel
+// kinds with a 512-thread instance of the stage kernel: Euler with the
+// limiter, every flux (the viscous and SA instances and the unlimited ones
+// keep 256 threads)
 bool has_wide_block(const fvm_params &q)
 {
-    return q.iflow == 1 && q.iflux == 1 && q.ilimit == 1;
+    return q.iflow == 1 && q.ilimit == 1;
 }
 
 void launch_stage(const StageLaunch &b, int irk, int mode)
@@ -2057,7 +2084,7 @@ void launch_stage(const StageLaunch &b, int irk, int mode)
               [&](auto fl, auto lm, auto vi, auto tu, auto pr) {
         constexpr int F = decltype(fl)::value, L = decltype(lm)::value, P = decltype(pr)::value;
         constexpr bool V = decltype(vi)::value != 0, T = decltype(tu)::value != 0;
-        if constexpr (F == 1 && L == 1 && !V && !T) {
+        if constexpr (L == 1 && !V && !T) {
             if (b.threads == 512) {
                 if (mode == 0)
                     hipLaunchKernelGGL((k_patch_batch<F, L, V, 0, T, P, false, 512>), dim3(b.nwork), dim3(512),
@@ -2096,7 +2123,7 @@ void launch_residual(fvm_ctx *x, DevState *S, const double4 *pin, double4 *res)
               [&](auto fl, auto lm, auto vi, auto tu, auto pr) {
         constexpr int F = decltype(fl)::value, L = decltype(lm)::value, P = decltype(pr)::value;
         constexpr bool V = decltype(vi)::value != 0, T = decltype(tu)::value != 0;
-        if constexpr (F == 1 && L == 1 && !V && !T) {
+        if constexpr (L == 1 && !V && !T) {
             if (x->threads == 512) {
                 hipLaunchKernelGGL((k_patch<F, L, V, 2, T, P, 512>), dim3(x->pa.npatch), dim3(512),
                                    x->pa.lds_bytes, x->stream, x->d, x->c, S, x->pa, 1, pin, x->pbuf[1], res);
@@ -2986,10 +3013,10 @@ int fvm_gpu_create(const fvm_params *par, const fvm_mesh *m, fvm_ctx **out)
                                 : (par->iflow == 2 ? kPatchWavesLaminar
                                                    : (par->precision == 1 ? kPatchWavesSplit : kPatchWavesEuler));
     // Workgroup size: the caller's, or with the automatic launch shape 512
-    // threads for Euler/Roe/limited solves of at least 32 768 vertices (two
+    // threads for limited Euler solves of at least 32 768 vertices (two
     // workgroups a CU on patches of about 130 vertices, fewer halo vertices
-    // per patch vertex: 513x129 FP64 41.8 against 45.0 us per iteration),
-    // else 256.
+    // per patch vertex: 513x129 FP64 with Roe 41.8 against 45.0 us per
+    // iteration; KFVS and HLLC stages 4-8% shorter), else 256.
     x->threads = par->block_threads ? par->block_threads
                                     : ((par->patch_size == 0 && has_wide_block(*par) && np >= 32768) ? 512 : kBlock);
     const int scale = x->threads / kBlock;          // 1, or 2 for 512-thread workgroups

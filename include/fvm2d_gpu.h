@@ -62,9 +62,9 @@ typedef struct {
     double nut_factor;  /* initial nu_tilde = nut_factor*mu_l/rho
                            (src/initialize.f:71 uses 0.1)                       */
     int block_threads;  /* workgroup size of the stage kernel: 256, or 512
-                           (Euler, Roe, limited only; patches up to 512
-                           vertices); 0 = automatic: 512 for Euler, Roe,
-                           limited solves of at least 32768 vertices with the
+                           (Euler with the limiter only, any flux; patches up
+                           to 512 vertices); 0 = automatic: 512 for limited
+                           Euler solves of at least 32768 vertices with the
                            automatic patch size, else 256.  The same results
                            either way; batch members share one size          */
     int reserved_params; /* 0                                                   */

@@ -32,7 +32,7 @@ def test_tune_picks_fastest_and_caches(tmp_path):
     c = autotune.tune(None, geo, inp, cache_path=path, device="gfx950", timer=timer)
     assert (c.threads, c.patch_size, c.xcd) == (256, 65, 1) and c.us_per_iter == 100.0
     assert len(calls) == 2 * len(autotune.shapes(66_048, inp))
-    assert {t for t, _, _ in calls} == {256, 512}          # Euler, Roe, limited: both sizes
+    assert {t for t, _, _ in calls} == {256, 512}          # Euler, limited: both sizes
     n = len(calls)
     again = autotune.tune(None, geo, inp, cache_path=path, device="gfx950", timer=timer)
     assert len(calls) == n and again.patch_size == 65          # from the cache
@@ -42,11 +42,12 @@ def test_tune_picks_fastest_and_caches(tmp_path):
 
 
 def test_block_sizes_per_kind():
-    """512-thread workgroups exist for Euler, Roe, limited only; their patch
-    candidates are about twice as large and fill whole rounds of two
-    workgroups a CU."""
-    assert autotune.block_sizes(Inputs(iflow=1, iflux=1, ilimit=1)) == [256, 512]
-    for kw in (dict(iflow=2), dict(iflow=3), dict(iflow=1, iflux=2), dict(iflow=1, ilimit=0)):
+    """512-thread workgroups exist for limited Euler solves (every flux)
+    only; their patch candidates are about twice as large and fill whole
+    rounds of two workgroups a CU."""
+    for iflux in (1, 2, 3):
+        assert autotune.block_sizes(Inputs(iflow=1, iflux=iflux, ilimit=1)) == [256, 512]
+    for kw in (dict(iflow=2), dict(iflow=3), dict(iflow=1, ilimit=0), dict(iflow=1, iflux=2, ilimit=0)):
         assert autotune.block_sizes(Inputs(**kw)) == [256]
     sh = autotune.shapes(66_048, Inputs(iflow=1))
     big = [ps for t, ps in sh if t == 512]

@@ -63,14 +63,14 @@ def test_host_program_built():
     assert "You must specify an input file." in r.stdout
 
 
-def test_512_thread_workgroups_only_for_euler_roe_limited():
+def test_512_thread_workgroups_only_for_limited_euler():
     """fvm_gpu_create refuses block_threads other than 0/256/512, and 512 for
     kinds without that instance, before touching the device."""
     from fvm2d_amd import mesh
     from fvm2d_amd.gpu import FvmError
     from fvm2d_amd.solver import Inputs, Solver
     g = mesh.o_grid(33, 9, seed=1)
-    for kw in (dict(block_threads=384), dict(block_threads=512, iflux=2),
-               dict(block_threads=512, iflow=2, rey=100.0), dict(block_threads=512, ilimit=0)):
+    for kw in (dict(block_threads=384), dict(block_threads=512, iflow=2, rey=100.0),
+               dict(block_threads=512, ilimit=0), dict(block_threads=512, iflux=2, ilimit=0)):
         with pytest.raises(FvmError):
             Solver(g, Inputs.from_kwargs(**kw))
