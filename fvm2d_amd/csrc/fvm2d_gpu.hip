@@ -294,13 +294,59 @@ __device__ __forceinline__ Red red_join(Red a, Red b)
     return r;
 }
 
-// Block reductions: a butterfly over the 64 lanes of each wavefront (every
-// lane ends with the wavefront's total), then the wavefront totals in order.
+// Block reductions: the 64 lanes of each wavefront joined in a fixed pattern
+// (every lane ends with the wavefront's total), then the wavefront totals in
+// order.
 // The pattern is fixed, so results repeat bit for bit; every thread returns
 // the total.
 __device__ __forceinline__ Red shfl_red(Red a, int o)
 {
     return Red{__shfl_xor(a.s, o), __shfl_xor(a.m, o), __shfl_xor(a.i, o)};
+}
+
+// A lane's partner value under a DPP pattern (all lanes active): the xor 1
+// and xor 2 quad permutations, the half-row and the row mirror.
+template <int CTRL>
+__device__ __forceinline__ int dpp_int(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double x)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)dpp_int<CTRL>((int)(unsigned)b);
+    const unsigned hi = (unsigned)dpp_int<CTRL>((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <int CTRL>
+__device__ __forceinline__ Red dpp_red(Red a)
+{
+    return Red{dpp_double<CTRL>(a.s), dpp_double<CTRL>(a.m), dpp_int<CTRL>(a.i)};
+}
+__device__ __forceinline__ Red lane_red(Red a, int l)
+{
+    const unsigned long long bs = __builtin_bit_cast(unsigned long long, a.s);
+    const unsigned long long bm = __builtin_bit_cast(unsigned long long, a.m);
+    auto rd = [&](unsigned long long b) {
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+        return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+    };
+    return Red{rd(bs), rd(bm), __builtin_amdgcn_readlane(a.i, l)};
+}
+
+// The total of a wavefront in every lane, without the LDS unit: lanes joined
+// in quads (xor 1, xor 2), then half rows and rows (mirrors; each join pairs
+// two groups that already hold equal values, and the join is commutative,
+// so all lanes of a group stay equal), then the four rows in order.
+__device__ __forceinline__ Red wave_reduce(Red x)
+{
+    x = red_join(x, dpp_red<0xB1>(x));      // quad_perm [1, 0, 3, 2]
+    x = red_join(x, dpp_red<0x4E>(x));      // quad_perm [2, 3, 0, 1]
+    x = red_join(x, dpp_red<0x141>(x));     // row_half_mirror
+    x = red_join(x, dpp_red<0x140>(x));     // row_mirror
+    return red_join(red_join(lane_red(x, 0), lane_red(x, 16)), red_join(lane_red(x, 32), lane_red(x, 48)));
 }
 
 // (w, lane: the thread's wave and its lane in the wave)
@@ -309,8 +355,7 @@ __device__ Red block_reduce(Red x, const int w, const int lane)
 {
     __shared__ double ss[NT / 64], sm[NT / 64];
     __shared__ int si[NT / 64];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x = red_join(x, shfl_red(x, o));
+    x = wave_reduce(x);
     if (lane == 0) { ss[w] = x.s; sm[w] = x.m; si[w] = x.i; }
     __syncthreads();
     Red r{ss[0], sm[0], si[0]};
