@@ -1172,12 +1172,17 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         }
         const int k1 = s_pptr[tl + 1];
         const double4 *fl = reinterpret_cast<const double4 *>(s_work);
+        // a term enters with the edge's orientation: r - f is r + (-f) (IEEE
+        // 754 defines the difference as the sum with the negated operand:
+        // the same value, rounding and sign of zero), so the flux's sign bit
+        // is flipped by the orientation bit and the term added, without a
+        // branch that the lanes of a wave would take both ways
         auto apply = [&](const int q, const double4 f) {
-            if (q & 1) {
-                r.v[0] = r.v[0] - f.x; r.v[1] = r.v[1] - f.y; r.v[2] = r.v[2] - f.z; r.v[3] = r.v[3] - f.w;
-            } else {
-                r.v[0] = r.v[0] + f.x; r.v[1] = r.v[1] + f.y; r.v[2] = r.v[2] + f.z; r.v[3] = r.v[3] + f.w;
-            }
+            const unsigned long long m = (unsigned long long)(q & 1) << 63;
+            auto sg = [m](double x) {
+                return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) ^ m);
+            };
+            r.v[0] = r.v[0] + sg(f.x); r.v[1] = r.v[1] + sg(f.y); r.v[2] = r.v[2] + sg(f.z); r.v[3] = r.v[3] + sg(f.w);
         };
         // two terms a step, read before either is added, added in order
         int k = s_pptr[tl];
