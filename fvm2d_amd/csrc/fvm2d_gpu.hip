@@ -57,7 +57,8 @@ constexpr int patch_waves()
     return TURB ? kPatchWavesSA : (VISC ? kPatchWavesLaminar : (PREC ? kPatchWavesSplit : kPatchWavesEuler));
 }
 constexpr int kHistCap = 4096;      // history rows per fvm_gpu_iterate chunk
-constexpr int kGraphIters = 8;      // iterations captured per graph
+constexpr int kGraphIters = 32;     // iterations captured per graph (8: 0.9 us more per
+                                     // 513x129 iteration, the graph's last monitors and launch)
 
 struct DevState {
     int iter, done, status, err_code, err_index, nhist, iresi;
