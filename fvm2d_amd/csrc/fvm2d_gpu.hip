@@ -57,7 +57,7 @@ constexpr int patch_waves()
     return TURB ? kPatchWavesSA : (VISC ? kPatchWavesLaminar : (PREC ? kPatchWavesSplit : kPatchWavesEuler));
 }
 constexpr int kHistCap = 4096;      // history rows per fvm_gpu_iterate chunk
-constexpr int kGraphIters = 32;     // iterations captured per graph (8: 0.9 us more per
+constexpr int kGraphIters = 64;     // iterations captured per graph (8: 0.9 us more per
                                      // 513x129 iteration, the graph's last monitors and launch)
 
 struct DevState {
@@ -350,8 +350,9 @@ __device__ __forceinline__ Red wave_reduce(Red x)
     return red_join(red_join(lane_red(x, 0), lane_red(x, 16)), red_join(lane_red(x, 32), lane_red(x, 48)));
 }
 
-// (w, lane: the thread's wave and its lane in the wave)
-template <int NT>
+// (w, lane: the thread's wave and its lane in the wave; LAST: the kernel's
+// last use of the wavefront slots, which then need no closing barrier)
+template <int NT, bool LAST = false>
 __device__ Red block_reduce(Red x, const int w, const int lane)
 {
     __shared__ double ss[NT / 64], sm[NT / 64];
@@ -362,7 +363,7 @@ __device__ Red block_reduce(Red x, const int w, const int lane)
     Red r{ss[0], sm[0], si[0]};
 #pragma unroll
     for (int k = 1; k < NT / 64; ++k) r = red_join(r, Red{ss[k], sm[k], si[k]});
-    __syncthreads();
+    if (!LAST) __syncthreads();
     return r;
 }
 
@@ -1245,7 +1246,7 @@ __device__ __forceinline__ void patch_body(const Dev &d, const Consts &c, DevSta
         }
     }
     if (MODE == 1) {
-        Red rr = block_reduce<WG>(red, wave, lane);
+        Red rr = block_reduce<WG, true>(red, wave, lane);
         if (tl == 0) {
             d.partial[3 * b] = rr.s;
             d.partial[3 * b + 1] = rr.m;

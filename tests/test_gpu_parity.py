@@ -129,7 +129,7 @@ def test_bitwise_run_to_run():
 
 
 def test_chunked_iterate_equals_one_call():
-    """Graph replays of 32 iterations + single iterations, in several calls."""
+    """Graph replays of 64 iterations + single iterations, in several calls."""
     name = "euler_roe_nolim_65x17"
     with solver_for(name) as a:
         ha, _ = a.iterate(45)
