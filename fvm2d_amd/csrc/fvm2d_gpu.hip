@@ -57,8 +57,10 @@ constexpr int patch_waves()
     return TURB ? kPatchWavesSA : (VISC ? kPatchWavesLaminar : (PREC ? kPatchWavesSplit : kPatchWavesEuler));
 }
 constexpr int kHistCap = 4096;      // history rows per fvm_gpu_iterate chunk
-constexpr int kGraphIters = 64;     // iterations captured per graph (8: 0.9 us more per
-                                     // 513x129 iteration, the graph's last monitors and launch)
+constexpr int kGraphIters = 32;     // iterations captured per graph (8: 0.9 us more per
+                                     // 513x129 iteration, the graph's last monitors and launch;
+                                     // 64: 0.1 us less, but a batch whose members change
+                                     // recaptures twice the nodes: 5% off 200-iteration requests)
 
 struct DevState {
     int iter, done, status, err_code, err_index, nhist, iresi;
